@@ -1,0 +1,149 @@
+"""Pins the oracle's random-number layer.
+
+Every value here comes from the reference's own engine
+(ModelG/xoroshiro128plus.h) driven by libstdc++ exactly as
+ModelG/NoiseGenerator.h drives it: the committed fixtures
+tests/golden/rng_golden.json (written by tests/golden/make_rng_golden.py from
+oracle/_ref/ref_rng) and, where the reference tree is present, fresh runs of
+ref_rng itself.
+"""
+import json
+import os
+import random
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "rng_golden.json")))
+
+
+def hexf(s):
+    return float.fromhex(s)
+
+
+def test_engine_matches_reference_header(orc):
+    for case in GOLD["engine"]:
+        e = orc.Engine(seed=case["seed"])
+        assert [e.next() for _ in case["out"]] == case["out"]
+
+
+def test_seed_seq_matches_libstdcxx(orc):
+    for case in GOLD["seedseq"]:
+        assert orc.seed_seq_generate([case["base"]], case["n"]) == case["out"], case
+
+
+def test_noise_generator_sequences(orc):
+    # NoiseGenerator(base): uniform(), variance1(), normal() interleaved;
+    # normal() pairs share one polar draw and the cached value.
+    for case in GOLD["noise"]:
+        ng = orc.NoiseGenerator(case["base"])
+        got = []
+        for ch in case["pattern"]:
+            got.append({"u": ng.uniform, "v": ng.variance1, "n": ng.normal}[ch]())
+        assert got == [hexf(x) for x in case["out"]], case["pattern"]
+
+
+def test_survey_probe_values(orc):
+    # seed_seq{42} -> 2934951935 and the first draws quoted for seed 42
+    assert orc.seed_seq_generate([42], 1) == [2934951935]
+    ng = orc.NoiseGenerator(42)
+    u = [ng.uniform() for _ in range(3)]
+    v = [ng.variance1() for _ in range(3)]
+    assert u == [0.58054516675970202, 0.28031185620362042, 0.57774961428734561]
+    assert v == [-1.5649486830557773, 1.2661790141300289, 0.73963848537230614]
+
+
+def test_shuffle_matches_std_shuffle(orc):
+    for case in GOLD["shuffle"]:
+        e = orc.Engine(state=case["state"])
+        for row in case["out"]:
+            assert e.shuffle([0, 1, 2]) + e.shuffle([0, 1]) == row
+
+
+def test_generate_canonical_clamp(orc):
+    c = GOLD["canon"]
+    for x, y in zip(c["in"], c["out"]):
+        assert orc.canonical(x) == hexf(y), hex(x)
+    # values that round up to 2^64 are clamped to the largest double below 1
+    assert orc.canonical(2**64 - 1) == float.fromhex("0x1.fffffffffffffp-1")
+
+
+def test_rng_state_text(orc):
+    # ModelG/NoiseGenerator.h:112-147 writes "s0 s1"; test_random.cxx:107-126
+    # saves after two draws and resumes; the resumed stream must continue.
+    for case in GOLD["state"]:
+        ng = orc.NoiseGenerator(case["base"])
+        for _ in range(case["ndraws"]):
+            ng.uniform()
+        s0, s1 = (int(x) for x in case["out"].split())
+        assert (ng.ng.g.s0, ng.ng.g.s1) == (s0, s1)
+    ng = orc.NoiseGenerator(2021)
+    a = [ng.uniform() for _ in range(4)]
+    ng2 = orc.NoiseGenerator(2021)
+    ng2.uniform(), ng2.uniform()
+    resumed = orc.Engine(state=(ng2.ng.g.s0, ng2.ng.g.s1))
+    assert [resumed.uniform(), resumed.uniform()] == a[2:]
+
+
+def test_variance1_is_unit_variance(orc):
+    e = orc.Engine(seed=7)
+    import numpy as np
+    x = np.array([e.variance1() for _ in range(200000)])
+    assert abs(x.mean()) < 0.01 and abs((x * x).mean() - 1.0) < 0.01
+    assert x.min() >= -3**0.5 and x.max() < 3**0.5
+
+
+def test_uniform_int_lemire_rejection(orc):
+    # uniform_int{0,b} draws again when the low word falls below
+    # (2^64 - range) % range; for b = 5 the threshold is 4.
+    # engine state chosen so that the first output is 0 (s0 + s1 == 0)
+    e = orc.Engine(state=(1, 2**64 - 1))
+    first = orc.Engine(state=(1, 2**64 - 1)).next()
+    assert first == 0
+    r = e.uniform_int(5)
+    assert 0 <= r <= 5
+    # the first draw (0 * 6, low = 0 < 4) was rejected, so two were consumed
+    e2 = orc.Engine(state=(1, 2**64 - 1))
+    e2.next(), e2.next()
+    assert e.state == e2.state
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(HERE, "..", "oracle", "_ref", "ref_rng")),
+                    reason="oracle/_ref not built (reference tree absent)")
+def test_random_states_against_live_reference(orc):
+    rnd = random.Random(1234)
+    for _ in range(20):
+        seed = rnd.getrandbits(64)
+        out = [int(x) for x in orc.ref_rng("engine", seed, 50)]
+        e = orc.Engine(seed=seed)
+        assert [e.next() for _ in range(50)] == out
+    for _ in range(10):
+        base = rnd.randint(-2**31, 2**31 - 1)
+        n = rnd.randint(1, 700)
+        assert orc.seed_seq_generate([base], n) == [int(x) for x in orc.ref_rng("seedseq", base, n)]
+        pat = "".join(rnd.choice("uvn") for _ in range(300))
+        ng = orc.NoiseGenerator(base)
+        got = [{"u": ng.uniform, "v": ng.variance1, "n": ng.normal}[ch]() for ch in pat]
+        assert got == [hexf(x) for x in orc.ref_rng("noise", base, pat)]
+    for _ in range(5):
+        s0, s1 = rnd.getrandbits(64), rnd.getrandbits(64)
+        rows = orc.ref_rng("shuffle", s0, s1, 200)
+        e = orc.Engine(state=(s0, s1))
+        for r in rows:
+            assert e.shuffle([0, 1, 2]) + e.shuffle([0, 1]) == [int(v) for v in r.split()]
+    words = [rnd.getrandbits(64) for _ in range(200)] + [2**64 - rnd.randint(1, 4096) for _ in range(200)]
+    for x, y in zip(words, orc.ref_rng("canon", *words)):
+        assert orc.canonical(x) == hexf(y)
+
+
+def test_site_keys_distinct_and_mixed(orc):
+    base = orc.base_key(orc.seed_seq_generate([42], 1)[0])
+    sk = orc.stream_key(base, 0)
+    keys = {orc.site_key(sk, s) for s in range(4096)}
+    assert len(keys) == 4096
+    # different counters give different streams for the same site
+    assert orc.site_key(orc.stream_key(base, 1), 0) != orc.site_key(sk, 0)
+    # the per-site stream is a plain reference engine seeded with the key
+    k = orc.site_key(sk, 17)
+    e = orc.Engine(seed=k)
+    assert e.next() == ((0x01d353e5f3993bb0 + k) + 0x7b9c0df6cb193b20 * (k + 1)) % 2**64
