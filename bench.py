@@ -1,0 +1,139 @@
+#!/usr/bin/env python3
+"""Throughput of the ModelG time step (PV2HBSplit23) on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--size 256] [--precision fp64|fp32]
+                    [--replicas batched] [--no-cpu-baseline]
+
+One "step" is PV2HBSplit::step(dt) (3 ideal substeps, 12 phi Metropolis
+half-sweeps, 6 charge sub-sweeps) applied to the whole lattice; the metric
+is site-updates/s = sites x steps / wall seconds, inputs resident in HBM.
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
+# algorithmic bytes per site per launch, FP64 fields (halve for FP32):
+# ideal substep read+write 80+80, phi half-sweep read 32 + write 16,
+# charge sub-sweep read 48 + write 48
+ALG_BYTES = {"ideal": 160, "phi": 48, "charge": 96}
+ALG_BYTES_PER_SITE_UPDATE = 3 * 160 + 12 * 48 + 6 * 96  # 1632
+
+PARAMS = dict(mass0=-4.813, lam=1.0, H=0.01, chi=2.0, gamma=1.0)
+DT = 0.04
+
+
+def cpu_baseline(seconds_target=12.0):
+    """The CPU oracle (a C restatement of the reference's stepper, one
+    thread) timed on this host on a bounded sample: 128^3 FP64 PV2HBSplit23
+    steps until ~seconds_target of work."""
+    from oracle import oracle as orc
+    N = 128
+    lat = orc.Lattice(N, mass0=PARAMS["mass0"], lam=PARAMS["lam"], H=PARAMS["H"],
+                      chi=PARAMS["chi"], seed=42, rng_mode=orc.RNG_PER_SITE)
+    lat.init_gaussian_charges()
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        lat.step(DT)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > seconds_target or steps >= 10:
+            break
+    return {"value": N ** 3 * steps / el, "unit": "site-updates/s", "cores": 1,
+            "kind": "port",
+            "sample": f"oracle/lgv_oracle.c, {steps} PV2HBSplit23 step(s) of a {N}^3 FP64 "
+                      f"lattice ({el:.1f} s, 1 thread)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--graph", action="store_true", help="replay a HIP graph per step")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the lattice and every replica "
+              "live on a single MI355X); --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import langevina_amd as L
+
+    N = args.size
+    prec = L.FP64 if args.precision == "fp64" else L.FP32
+    esz = 8 if prec == L.FP64 else 4
+    ctx = L.Context(L.model_params(N, seed=42, **PARAMS), precision=prec)
+    # synthetic start: phi = 0 (ModelA::initialize), Gaussian charges with the
+    # zero mode removed (ModelA::initialize_gaussian_charges)
+    ctx.init_gaussian_charges()
+    if args.graph:
+        ctx.use_graph(True)
+    ctx.run(DT, args.warmup)
+    ctx.synchronize()
+
+    ctx.timing_enable(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.run(DT, args.steps)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    tm = ctx.timing()
+    ctx.timing_enable(False)
+
+    sites = N ** 3
+    value = sites * args.steps / el
+    ms_per_step = el * 1e3 / args.steps
+    # dominant kernel class: the one with the largest total time
+    dom = max(("ideal", "phi", "charge"), key=lambda k: tm[k][0])
+    ms, launches = tm[dom]
+    per_launch_s = ms * 1e-3 / max(launches, 1)
+    bytes_per_launch = ALG_BYTES[dom] * sites * esz // 8
+    achieved = bytes_per_launch / per_launch_s
+    kernels = {k: {"ms_total": round(v[0], 3), "launches": v[1],
+                   "avg_us": round(v[0] * 1e3 / max(v[1], 1), 2),
+                   "alg_GBps": round(ALG_BYTES[k] * sites * esz / 8 /
+                                     (v[0] * 1e-3 / max(v[1], 1)) / 1e9, 1) if v[1] else None}
+               for k, v in tm.items() if k in ALG_BYTES}
+    out = {
+        "metric": "site-updates/sec (ModelG 256^3 FP64 vs FP32/batched) and % HBM roofline, 1 GPU",
+        "value": value,
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == L.FP64 else "f32-storage/f64-arith",
+        "data": "synthetic (phi=0, Gaussian charges, zero mode removed; seed 42)",
+        "config": {"workload": f"ModelG {N}^3 PV2HBSplit23 "
+                               f"{'FP64' if prec == L.FP64 else 'FP32 fields'}",
+                   "N": N, "dt": DT, "params": PARAMS, "parallelism": "1 GPU",
+                   "graph": bool(args.graph)},
+        "step_roofline_frac": value * ALG_BYTES_PER_SITE_UPDATE * esz / 8 / HBM_PEAK,
+        "roofline": {"bound": "hbm", "kernel": {"ideal": "k_ideal_rot1+k_ideal_kick_rot2",
+                                                 "phi": "k_hb_phi", "charge": "k_charge"}[dom],
+                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": per_launch_s * 1e6},
+        "kernels": kernels,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,159 @@
+/*
+ * lgv.h -- C ABI of the MI355X ModelG hot path (liblgv_hip.so).
+ *
+ * Plain pointers and sizes only; every call returns LGV_OK (0) or a negative
+ * status, with a message in lgv_last_error().  No exception crosses this
+ * boundary.  One context drives one GPU and holds one or more independent
+ * replicas (lattices with their own size, seed and couplings) that are
+ * stepped together.
+ *
+ * Reference interface replaced (paths relative to the reference tree):
+ *   Stepper::step / finalize                    ModelG/Stepper.h:6-11
+ *   PV2HBSplit (scounts, nodiffuse, onlydiffuse) ModelG/Stepper.h:176-211
+ *   IdealPV2 / EulerLangevinHB / ModelGChargeHB ModelG/Stepper.h:87-154
+ *   ModelAData / ModelACoefficients (params)    ModelG/ModelA.h:71-111,185-269
+ *   ModelA::solution (PETSc Vec of G_node)      ModelG/ModelA.h:274-336
+ *   ModelA::initialize_gaussian_charges         ModelG/ModelA.h:463-525
+ *   NoiseGenerator / ModelARndm                 ModelG/NoiseGenerator.h:62-160
+ *   Measurer::measure (walls, OAverage)         ModelG/measurer.h:169-179
+ *   o4_stepper_monitor                          ModelG/Stepper.h:17-83
+ */
+#ifndef LGV_H
+#define LGV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LGV_OK 0
+#define LGV_ERR_ARG (-1)
+#define LGV_ERR_HIP (-2)
+#define LGV_ERR_STATE (-3)
+#define LGV_ERR_RANGE (-4)
+
+enum { LGV_FP64 = 0, LGV_FP32 = 1 };
+
+/* ModelAData + ModelACoefficients + the seed of ModelAHandlerData
+ * (ModelG/ModelA.h:71-111, 114-181, 185-248).  NY = NZ = NX as in the
+ * reference; LX is read as an integer there (ModelA.h:229), 0 means LX = NX. */
+typedef struct {
+  int NX;
+  int LX;
+  double mass0, dmassdt;
+  double lambda, H, chi;
+  double gamma, diffusion;
+  int seed;
+  double initialtime;
+} lgv_model_params;
+
+typedef struct {
+  int nreplicas;
+  const lgv_model_params *replicas;
+  int precision;        /* LGV_FP64 or LGV_FP32 (fields stored in FP32, all
+                           arithmetic and reductions in FP64) */
+  unsigned scounts[2];  /* PV2HBSplit {inner, outer}; {2,3} = PV2HBSplit23 */
+  int nodiffuse;        /* PV2HBSplit23NoDiffuse  (SuperPions.cxx:212-215) */
+  int onlydiffuse;      /* PV2HBSplit23OnlyDiffuse (SuperPions.cxx:216-220) */
+  int track_energy;     /* compute IdealPV2's E_old/E_new and its monitor */
+  int device;           /* HIP device ordinal */
+} lgv_config;
+
+/* o4_stepper_monitor, ModelG/Stepper.h:17-83 */
+typedef struct {
+  int64_t down_counts, up_counts, up_yes, up_no;
+  double down_dS, up_dS;
+} lgv_monitor;
+
+enum { LGV_MON_PHI = 0, LGV_MON_CHARGE = 1, LGV_MON_IDEAL = 2 };
+
+typedef struct lgv_ctx lgv_ctx;
+
+const char *lgv_last_error(void);
+int lgv_version(void);
+
+/* ModelA::ModelA (grid + NoiseGenerator(seed)), ModelG/ModelA.h:311-336,
+ * and the PV2HBSplit constructor, ModelG/Stepper.cxx:537-540.  Fields start
+ * at zero (ModelA::initialize, ModelA.h:408-461). */
+int lgv_create(const lgv_config *cfg, lgv_ctx **out);
+/* Stepper::finalize + ModelA::finalize storage release. */
+int lgv_destroy(lgv_ctx *ctx);
+
+/* ModelA::solution in the o4fields layout [z][y][x][10] (G_node order
+ * f[4], A[3], V[3]), FP64 host memory (ModelA.h:274-281, 384-402). */
+int lgv_set_field(lgv_ctx *ctx, int replica, const double *aos);
+int lgv_get_field(lgv_ctx *ctx, int replica, double *aos);
+
+/* ModelATime::time (ModelA.h:32-68): the clock that m^2(t) reads.
+ * replica < 0 sets every replica. */
+int lgv_set_time(lgv_ctx *ctx, int replica, double t);
+int lgv_get_time(lgv_ctx *ctx, int replica, double *t);
+
+/* PV2HBSplit::step(dt), ModelG/Stepper.cxx:542-580, on every replica.
+ * Stream-ordered; does not advance the clock (the caller does, as
+ * run_event does at SuperPions.cxx:158-159). */
+int lgv_step(lgv_ctx *ctx, double dt);
+/* nsteps x { PV2HBSplit::step(dt); t += dt } -- the inner loop of
+ * run_event (SuperPions.cxx:124-160) without the measurements. */
+int lgv_run(lgv_ctx *ctx, double dt, int nsteps);
+/* Enables a HIP graph of one full step that lgv_run replays (0 = off). */
+int lgv_use_graph(lgv_ctx *ctx, int enable);
+
+/* The sub-steppers on their own: IdealPV2::step (Stepper.cxx:23-66),
+ * EulerLangevinHB::step (Stepper.cxx:287-389), ModelGChargeHB::step
+ * (Stepper.cxx:449-522). */
+int lgv_ideal_step(lgv_ctx *ctx, double dt);
+int lgv_hb_phi_step(lgv_ctx *ctx, double dt);
+int lgv_charge_step(lgv_ctx *ctx, double dt);
+
+/* ModelA::initialize_gaussian_charges (ModelA.h:463-525): normal() pairs
+ * from the replica's global engine, zero mode removed; host-generated. */
+int lgv_init_gaussian_charges(lgv_ctx *ctx, int replica);
+/* thermalize_event (SuperPions.cxx:21-76) without quench: charges, then
+ * 6*nsteps phi sweeps at dt/6 with the clock held. */
+int lgv_thermalize(lgv_ctx *ctx, double dt, int nsteps);
+
+/* o4_stepper_monitor counters of one replica: out[LGV_MON_PHI],
+ * out[LGV_MON_CHARGE], out[LGV_MON_IDEAL]. */
+int lgv_monitor_get(lgv_ctx *ctx, int replica, lgv_monitor out[3]);
+/* IdealPV2::getEnergy (Stepper.h:98-101): the last substep's E_old, E_new. */
+int lgv_energy_get(lgv_ctx *ctx, int replica, double out[2]);
+
+/* Random-number state: the replica's global engine (the two words that
+ * NoiseGenerator::write stores in tag_<rank>.rng, NoiseGenerator.h:112-147),
+ * the cached normal, and the per-site stream counter. */
+typedef struct {
+  uint64_t s0, s1;
+  double saved_normal;
+  int has_saved_normal;
+  uint64_t sweep_ctr;
+} lgv_rng_state;
+int lgv_rng_get_state(lgv_ctx *ctx, int replica, lgv_rng_state *st);
+int lgv_rng_set_state(lgv_ctx *ctx, int replica, const lgv_rng_state *st);
+
+/* Measurer::computeSliceAverage / computeSliceAveragePhase and the
+ * OAverage of computeDerivedObs (measurer.cxx:10-300) for one replica, in
+ * FP64 with compensated sums: wall[3][11][N], wallphase[3][14][N]
+ * (d = x, y, z; already divided by N^2), oavg[11]. */
+int lgv_measure(lgv_ctx *ctx, int replica, double *wall, double *wallphase,
+                double *oavg);
+
+/* HIP-event timing of the kernel classes of a step on the context's stream
+ * (the reference's PetscLogEvent "IdealStep" / "HBStep" / "QHBStep",
+ * Stepper.cxx:547-577).  Enabling resets the totals; while enabled, lgv_run
+ * issues kernels directly instead of replaying a graph. */
+enum { LGV_CLASS_IDEAL = 0, LGV_CLASS_PHI = 1, LGV_CLASS_CHARGE = 2, LGV_CLASS_ENERGY = 3,
+       LGV_NCLASS = 4 };
+int lgv_timing_enable(lgv_ctx *ctx, int enable);
+int lgv_timing_get(lgv_ctx *ctx, double ms[LGV_NCLASS], long long launches[LGV_NCLASS]);
+
+int lgv_synchronize(lgv_ctx *ctx);
+/* Device address of a replica's field storage (10 component planes, each
+ * split into even and odd half-lattices) and its element size in bytes. */
+int lgv_field_device_ptr(lgv_ctx *ctx, int replica, void **ptr, int *elem_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,688 @@
+// lgv_api.hip -- the C ABI of include/lgv.h: context, replica storage,
+// composition of PV2HBSplit::step from the kernels, host-side global engine
+// (initial charges), monitors and measurement.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "lgv.h"
+#include "lgv_device.h"
+#include "lgv_internal.h"
+
+using namespace lgv;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHECK(expr)                                                          \
+  do {                                                                          \
+    hipError_t e_ = (expr);                                                     \
+    if (e_ != hipSuccess)                                                       \
+      return fail(LGV_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                          \
+  } while (0)
+
+// std::seed_seq{base}.generate(1 word): the seed NoiseGenerator gives rank 0
+// (ModelG/NoiseGenerator.h:68-82).
+uint32_t rank0_seed(int base) {
+  std::seed_seq seq = {base};
+  uint32_t out = 0;
+  seq.generate(&out, &out + 1);
+  return out;
+}
+
+// NoiseGenerator's engine and its normal() (libstdc++ 11 Marsaglia polar
+// method with one cached value, bits/random.tcc:1802-1835).
+struct HostNoise {
+  Xoro g;
+  double saved = 0.;
+  int has_saved = 0;
+  double normal() {
+    if (has_saved) {
+      has_saved = 0;
+      return saved * 1.0 + 0.0;
+    }
+    double x, y, r2;
+    do {
+      x = 2.0 * canonical(xoro_next(g)) - 1.0;
+      y = 2.0 * canonical(xoro_next(g)) - 1.0;
+      r2 = x * x + y * y;
+    } while (r2 > 1.0 || r2 == 0.0);
+    const double mult = std::sqrt(-2 * std::log(r2) / r2);
+    saved = x * mult;
+    has_saved = 1;
+    return (y * mult) * 1.0 + 0.0;
+  }
+};
+
+}  // namespace
+
+struct lgv_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool fp32 = false;
+  unsigned s0 = 2, s1 = 3;
+  bool nodiffuse = false, onlydiffuse = false, track_energy = false;
+  int nrep = 0;
+  std::vector<lgv_model_params> mp;
+  std::vector<ReplicaParams> hp;
+  std::vector<HostNoise> noise;
+  ReplicaParams *dP = nullptr;
+  ReplicaState *dS = nullptr;
+  std::vector<void *> fields;
+  std::vector<double *> scratch;
+  Plan plan_half, plan_full;
+  MonSlot *mon_phi = nullptr, *mon_charge = nullptr, *mon_ideal = nullptr;
+  EnergySlot *eslots = nullptr;
+  double coef_hb = NAN, coef_charge = NAN, coef_ideal = NAN;
+  double *d_aos = nullptr;
+  int maxN = 0;
+  double *d_partXZ = nullptr, *d_partXY = nullptr, *d_wall = nullptr, *d_wallphase = nullptr;
+  bool use_graph = false;
+  hipGraphExec_t graph_exec = nullptr;
+  double graph_dt = NAN;
+  // HIP-event timing of kernel classes on the context stream
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pairs;
+  double t_ms[LGV_NCLASS] = {0};
+  long long t_launch[LGV_NCLASS] = {0};
+};
+
+namespace {
+
+int build_plan(Plan &plan, const std::vector<int64_t> &work) {
+  std::vector<int> rep, first;
+  for (size_t r = 0; r < work.size(); r++) {
+    first.push_back((int)rep.size());
+    const int64_t nb = (work[r] + kBlock - 1) / kBlock;
+    for (int64_t b = 0; b < nb; b++) rep.push_back((int)r);
+  }
+  plan.nblocks = (int)rep.size();
+  HIPCHECK(hipMalloc(&plan.d_blk_rep, rep.size() * sizeof(int)));
+  HIPCHECK(hipMalloc(&plan.d_blk_first, first.size() * sizeof(int)));
+  HIPCHECK(hipMemcpy(plan.d_blk_rep, rep.data(), rep.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(plan.d_blk_first, first.data(), first.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
+  return LGV_OK;
+}
+
+// Step coefficients with the reference's expressions:
+//   EulerLangevinHB  rdtg = sqrt(2.*dt*gamma)          Stepper.cxx:301
+//   ModelGChargeHB   rdtsigma = sqrt(2.*dt*sigma())    Stepper.cxx:456
+//   IdealPV2         rotatePhi(dt/2.0)                 Stepper.cxx:73,169
+bool same(double a, double b) { return a == b || (std::isnan(a) && std::isnan(b)); }
+
+int ensure_coeffs(lgv_ctx *c, double dt_hb, double dt_charge, double dt_ideal) {
+  if (same(dt_hb, c->coef_hb) && same(dt_charge, c->coef_charge) &&
+      same(dt_ideal, c->coef_ideal))
+    return LGV_OK;
+  for (int r = 0; r < c->nrep; r++) {
+    const lgv_model_params &m = c->mp[r];
+    ReplicaParams &p = c->hp[r];
+    p.rdtg = std::sqrt(2. * dt_hb * m.gamma);
+    p.rms = std::sqrt(2. * dt_charge * (m.diffusion * m.chi));
+    p.dt_ideal = dt_ideal;
+    p.tau_ideal = dt_ideal / 2.0;
+  }
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  HIPCHECK(hipMemcpy(c->dP, c->hp.data(), c->nrep * sizeof(ReplicaParams),
+                     hipMemcpyHostToDevice));
+  c->coef_hb = dt_hb;
+  c->coef_charge = dt_charge;
+  c->coef_ideal = dt_ideal;
+  return LGV_OK;
+}
+
+// Brackets one kernel (class cls) with events when timing is on.
+struct Timed {
+  lgv_ctx *c;
+  int cls;
+  hipEvent_t e1 = nullptr;
+  Timed(lgv_ctx *cc, int k) : c(cc), cls(k) {
+    if (!c->timing) return;
+    if (c->ev_used + 2 > c->ev_pool.size()) {
+      for (int i = 0; i < 64; i++) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) { c->timing = false; return; }
+        c->ev_pool.push_back(e);
+      }
+    }
+    hipEvent_t e0 = c->ev_pool[c->ev_used++];
+    e1 = c->ev_pool[c->ev_used++];
+    hipEventRecord(e0, c->stream);
+    c->ev_pairs.push_back({cls, {e0, e1}});
+  }
+  ~Timed() {
+    if (c->timing && e1) hipEventRecord(e1, c->stream);
+  }
+};
+
+// Resolves recorded event pairs into the per-class totals.
+int drain_timing(lgv_ctx *c) {
+  if (c->ev_pairs.empty()) return LGV_OK;
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  for (auto &pr : c->ev_pairs) {
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, pr.second.first, pr.second.second));
+    c->t_ms[pr.first] += ms;
+    c->t_launch[pr.first] += 1;
+  }
+  c->ev_pairs.clear();
+  c->ev_used = 0;
+  return LGV_OK;
+}
+
+void enqueue_ideal(lgv_ctx *c, int off) {
+  if (c->timing && c->ev_used + 16 > 4096) drain_timing(c);
+  if (c->track_energy) {
+    Timed tm(c, LGV_CLASS_ENERGY);
+    launch_energy(c->fp32, c->plan_full, c->dP, c->eslots, c->stream);
+    launch_energy_finish(c->nrep, c->plan_full, c->dP, c->dS, c->eslots, 0, off,
+                         c->mon_ideal, c->stream);
+  }
+  {
+    Timed tm(c, LGV_CLASS_IDEAL);
+    launch_ideal_rot1(c->fp32, c->plan_full, c->dP, c->stream);
+    launch_ideal_kick_rot2(c->fp32, c->plan_full, c->dP, c->stream);
+  }
+  if (c->track_energy) {
+    Timed tm(c, LGV_CLASS_ENERGY);
+    launch_energy(c->fp32, c->plan_full, c->dP, c->eslots, c->stream);
+    launch_energy_finish(c->nrep, c->plan_full, c->dP, c->dS, c->eslots, 1, off,
+                         c->mon_ideal, c->stream);
+  }
+}
+
+void enqueue_hb(lgv_ctx *c, int off) {
+  for (int ieo = 0; ieo < 2; ieo++) {
+    Timed tm(c, LGV_CLASS_PHI);
+    launch_hb_phi(c->fp32, c->plan_half, c->dP, c->dS, ieo, off + ieo, c->mon_phi, c->stream);
+  }
+}
+
+void enqueue_charge(lgv_ctx *c, int off) {
+  launch_shuffle(c->nrep, c->dS, c->dP, off, c->stream);
+  for (int sub = 0; sub < 6; sub++) {
+    Timed tm(c, LGV_CLASS_CHARGE);
+    launch_charge(c->fp32, c->plan_half, c->dP, c->dS, sub, off + 1 + sub, c->mon_charge,
+                  c->stream);
+  }
+}
+
+// PV2HBSplit::step (Stepper.cxx:542-580); returns the number of stream
+// counters consumed.
+int enqueue_step(lgv_ctx *c) {
+  int off = 0;
+  if (!c->onlydiffuse) {
+    for (unsigned i1 = 0; i1 < c->s1; i1++) {
+      enqueue_ideal(c, off);
+      off += 1;
+      for (unsigned i0 = 0; i0 < c->s0; i0++) {
+        enqueue_hb(c, off);
+        off += 2;
+      }
+    }
+  }
+  if (!c->nodiffuse) {
+    enqueue_charge(c, off);
+    off += 7;
+  }
+  return off;
+}
+
+int check_launch() {
+  HIPCHECK(hipGetLastError());
+  return LGV_OK;
+}
+
+int check_replica(lgv_ctx *c, int r) {
+  if (!c) return fail(LGV_ERR_ARG, "null context");
+  if (r < 0 || r >= c->nrep) return fail(LGV_ERR_RANGE, "replica %d out of range [0,%d)", r, c->nrep);
+  return LGV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lgv_last_error(void) { return g_err.c_str(); }
+int lgv_version(void) { return 1; }
+
+int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
+  if (!cfg || !out) return fail(LGV_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (cfg->nreplicas <= 0 || !cfg->replicas) return fail(LGV_ERR_ARG, "no replicas");
+  if (cfg->precision != LGV_FP64 && cfg->precision != LGV_FP32)
+    return fail(LGV_ERR_ARG, "precision must be LGV_FP64 or LGV_FP32");
+  const unsigned s0 = cfg->scounts[0] ? cfg->scounts[0] : 2;
+  const unsigned s1 = cfg->scounts[1] ? cfg->scounts[1] : 3;
+  if (s0 > 64 || s1 > 64) return fail(LGV_ERR_ARG, "scounts too large");
+  for (int r = 0; r < cfg->nreplicas; r++) {
+    const lgv_model_params &m = cfg->replicas[r];
+    if (m.NX < 4 || (m.NX & 1) || m.NX > 2048)
+      return fail(LGV_ERR_ARG, "replica %d: NX=%d must be even and in [4, 2048]", r, m.NX);
+    if (!(m.chi > 0)) return fail(LGV_ERR_ARG, "replica %d: chi must be positive", r);
+  }
+  std::unique_ptr<lgv_ctx> c(new lgv_ctx);
+  c->device = cfg->device;
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  c->fp32 = cfg->precision == LGV_FP32;
+  c->s0 = s0;
+  c->s1 = s1;
+  c->nodiffuse = cfg->nodiffuse != 0;
+  c->onlydiffuse = cfg->onlydiffuse != 0;
+  c->track_energy = cfg->track_energy != 0;
+  c->nrep = cfg->nreplicas;
+  c->mp.assign(cfg->replicas, cfg->replicas + cfg->nreplicas);
+  c->hp.resize(c->nrep);
+  c->noise.resize(c->nrep);
+  const size_t esz = c->fp32 ? sizeof(float) : sizeof(double);
+  std::vector<int64_t> whalf, wfull;
+  std::vector<ReplicaState> hs(c->nrep);
+  for (int r = 0; r < c->nrep; r++) {
+    const lgv_model_params &m = c->mp[r];
+    const int N = m.NX;
+    const int64_t half = (int64_t)N * N * N / 2;
+    void *field = nullptr;
+    double *scr = nullptr;
+    HIPCHECK(hipMalloc(&field, 20 * half * esz));
+    HIPCHECK(hipMemset(field, 0, 20 * half * esz));
+    HIPCHECK(hipMalloc(&scr, 8 * half * sizeof(double)));
+    c->fields.push_back(field);
+    c->scratch.push_back(scr);
+    ReplicaParams &p = c->hp[r];
+    memset(&p, 0, sizeof p);
+    p.field = field;
+    p.scratch = scr;
+    p.N = N;
+    p.NH = N / 2;
+    p.half = half;
+    p.mass0 = m.mass0;
+    p.dmassdt = m.dmassdt;
+    p.lambda = m.lambda;
+    p.H = m.H;
+    p.chi = m.chi;
+    const double hx = (double)(m.LX > 0 ? m.LX : N) / N;
+    p.axx = std::pow(1. / hx, 2);
+    const uint32_t s32 = rank0_seed(m.seed);
+    p.base_key = base_key(s32);
+    c->noise[r].g = xoro_seeded((uint64_t)s32);
+    hs[r] = ReplicaState{};
+    hs[r].t = m.initialtime;
+    hs[r].order[0] = 0; hs[r].order[1] = 1; hs[r].order[2] = 2;
+    hs[r].order[3] = 0; hs[r].order[4] = 1;
+    whalf.push_back(half);
+    wfull.push_back(2 * half);
+    c->maxN = std::max(c->maxN, N);
+  }
+  HIPCHECK(hipMalloc(&c->dP, c->nrep * sizeof(ReplicaParams)));
+  HIPCHECK(hipMemcpy(c->dP, c->hp.data(), c->nrep * sizeof(ReplicaParams),
+                     hipMemcpyHostToDevice));
+  HIPCHECK(hipMalloc(&c->dS, c->nrep * sizeof(ReplicaState)));
+  HIPCHECK(hipMemcpy(c->dS, hs.data(), c->nrep * sizeof(ReplicaState), hipMemcpyHostToDevice));
+  int rc;
+  if ((rc = build_plan(c->plan_half, whalf)) != LGV_OK) return rc;
+  if ((rc = build_plan(c->plan_full, wfull)) != LGV_OK) return rc;
+  HIPCHECK(hipMalloc(&c->mon_phi, c->plan_half.nblocks * sizeof(MonSlot)));
+  HIPCHECK(hipMemset(c->mon_phi, 0, c->plan_half.nblocks * sizeof(MonSlot)));
+  HIPCHECK(hipMalloc(&c->mon_charge, c->plan_half.nblocks * sizeof(MonSlot)));
+  HIPCHECK(hipMemset(c->mon_charge, 0, c->plan_half.nblocks * sizeof(MonSlot)));
+  HIPCHECK(hipMalloc(&c->mon_ideal, c->nrep * sizeof(MonSlot)));
+  HIPCHECK(hipMemset(c->mon_ideal, 0, c->nrep * sizeof(MonSlot)));
+  HIPCHECK(hipMalloc(&c->eslots, c->plan_full.nblocks * sizeof(EnergySlot)));
+  const int64_t maxV = (int64_t)c->maxN * c->maxN * c->maxN;
+  HIPCHECK(hipMalloc(&c->d_aos, maxV * 10 * sizeof(double)));
+  HIPCHECK(hipMalloc(&c->d_partXZ, (int64_t)25 * c->maxN * c->maxN * sizeof(double)));
+  HIPCHECK(hipMalloc(&c->d_partXY, (int64_t)25 * c->maxN * c->maxN * sizeof(double)));
+  HIPCHECK(hipMalloc(&c->d_wall, 3 * 11 * c->maxN * sizeof(double)));
+  HIPCHECK(hipMalloc(&c->d_wallphase, 3 * 14 * c->maxN * sizeof(double)));
+  HIPCHECK(hipDeviceSynchronize());
+  *out = c.release();
+  return LGV_OK;
+}
+
+int lgv_destroy(lgv_ctx *c) {
+  if (!c) return LGV_OK;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+  for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+  for (void *f : c->fields) hipFree(f);
+  for (double *s : c->scratch) hipFree(s);
+  hipFree(c->dP);
+  hipFree(c->dS);
+  hipFree(c->plan_half.d_blk_rep);
+  hipFree(c->plan_half.d_blk_first);
+  hipFree(c->plan_full.d_blk_rep);
+  hipFree(c->plan_full.d_blk_first);
+  hipFree(c->mon_phi);
+  hipFree(c->mon_charge);
+  hipFree(c->mon_ideal);
+  hipFree(c->eslots);
+  hipFree(c->d_aos);
+  hipFree(c->d_partXZ);
+  hipFree(c->d_partXY);
+  hipFree(c->d_wall);
+  hipFree(c->d_wallphase);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return LGV_OK;
+}
+
+int lgv_set_field(lgv_ctx *c, int r, const double *aos) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  if (!aos) return fail(LGV_ERR_ARG, "null field");
+  HIPCHECK(hipSetDevice(c->device));
+  const int N = c->hp[r].N;
+  const int64_t V = (int64_t)N * N * N;
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  HIPCHECK(hipMemcpy(c->d_aos, aos, V * 10 * sizeof(double), hipMemcpyHostToDevice));
+  launch_aos_to_rb(c->fp32, c->dP, r, N, c->d_aos, c->stream);
+  if ((rc = check_launch())) return rc;
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return LGV_OK;
+}
+
+int lgv_get_field(lgv_ctx *c, int r, double *aos) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  if (!aos) return fail(LGV_ERR_ARG, "null field");
+  HIPCHECK(hipSetDevice(c->device));
+  const int N = c->hp[r].N;
+  const int64_t V = (int64_t)N * N * N;
+  launch_rb_to_aos(c->fp32, c->dP, r, N, c->d_aos, c->stream);
+  if ((rc = check_launch())) return rc;
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  HIPCHECK(hipMemcpy(aos, c->d_aos, V * 10 * sizeof(double), hipMemcpyDeviceToHost));
+  return LGV_OK;
+}
+
+int lgv_set_time(lgv_ctx *c, int r, double t) {
+  if (!c) return fail(LGV_ERR_ARG, "null context");
+  if (r >= c->nrep) return fail(LGV_ERR_RANGE, "replica out of range");
+  HIPCHECK(hipSetDevice(c->device));
+  launch_set_time(c->nrep, r, c->dS, t, c->stream);
+  return check_launch();
+}
+
+int lgv_get_time(lgv_ctx *c, int r, double *t) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  HIPCHECK(hipSetDevice(c->device));
+  ReplicaState s;
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  HIPCHECK(hipMemcpy(&s, c->dS + r, sizeof s, hipMemcpyDeviceToHost));
+  *t = s.t;
+  return LGV_OK;
+}
+
+int lgv_step(lgv_ctx *c, double dt) {
+  if (!c) return fail(LGV_ERR_ARG, "null context");
+  HIPCHECK(hipSetDevice(c->device));
+  int rc = ensure_coeffs(c, dt / (c->s0 * c->s1), dt, dt / (c->s1));
+  if (rc) return rc;
+  const int n = enqueue_step(c);
+  launch_advance(c->nrep, c->dS, dt, 0, (uint64_t)n, c->stream);
+  return check_launch();
+}
+
+int lgv_use_graph(lgv_ctx *c, int enable) {
+  if (!c) return fail(LGV_ERR_ARG, "null context");
+  c->use_graph = enable != 0;
+  return LGV_OK;
+}
+
+int lgv_run(lgv_ctx *c, double dt, int nsteps) {
+  if (!c) return fail(LGV_ERR_ARG, "null context");
+  if (nsteps < 0) return fail(LGV_ERR_ARG, "nsteps < 0");
+  HIPCHECK(hipSetDevice(c->device));
+  int rc = ensure_coeffs(c, dt / (c->s0 * c->s1), dt, dt / (c->s1));
+  if (rc) return rc;
+  if (!c->use_graph || c->timing) {
+    for (int i = 0; i < nsteps; i++) {
+      const int n = enqueue_step(c);
+      launch_advance(c->nrep, c->dS, dt, 1, (uint64_t)n, c->stream);
+    }
+    return check_launch();
+  }
+  if (!c->graph_exec || c->graph_dt != dt) {
+    if (c->graph_exec) {
+      HIPCHECK(hipGraphExecDestroy(c->graph_exec));
+      c->graph_exec = nullptr;
+    }
+    hipGraph_t graph;
+    HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int n = enqueue_step(c);
+    launch_advance(c->nrep, c->dS, dt, 1, (uint64_t)n, c->stream);
+    HIPCHECK(hipStreamEndCapture(c->stream, &graph));
+    HIPCHECK(hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0));
+    HIPCHECK(hipGraphDestroy(graph));
+    c->graph_dt = dt;
+  }
+  for (int i = 0; i < nsteps; i++) HIPCHECK(hipGraphLaunch(c->graph_exec, c->stream));
+  return check_launch();
+}
+
+int lgv_ideal_step(lgv_ctx *c, double dt) {
+  if (!c) return fail(LGV_ERR_ARG, "null context");
+  HIPCHECK(hipSetDevice(c->device));
+  int rc = ensure_coeffs(c, c->coef_hb, c->coef_charge, dt);
+  if (rc) return rc;
+  enqueue_ideal(c, 0);
+  launch_advance(c->nrep, c->dS, 0., 0, 1, c->stream);
+  return check_launch();
+}
+
+int lgv_hb_phi_step(lgv_ctx *c, double dt) {
+  if (!c) return fail(LGV_ERR_ARG, "null context");
+  HIPCHECK(hipSetDevice(c->device));
+  int rc = ensure_coeffs(c, dt, c->coef_charge, c->coef_ideal);
+  if (rc) return rc;
+  enqueue_hb(c, 0);
+  launch_advance(c->nrep, c->dS, 0., 0, 2, c->stream);
+  return check_launch();
+}
+
+int lgv_charge_step(lgv_ctx *c, double dt) {
+  if (!c) return fail(LGV_ERR_ARG, "null context");
+  HIPCHECK(hipSetDevice(c->device));
+  int rc = ensure_coeffs(c, c->coef_hb, dt, c->coef_ideal);
+  if (rc) return rc;
+  enqueue_charge(c, 0);
+  launch_advance(c->nrep, c->dS, 0., 0, 7, c->stream);
+  return check_launch();
+}
+
+int lgv_init_gaussian_charges(lgv_ctx *c, int r) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  const int N = c->hp[r].N;
+  const size_t V = (size_t)N * N * N;
+  std::vector<double> u(V * 10);
+  if ((rc = lgv_get_field(c, r, u.data()))) return rc;
+  double sum[10] = {0};
+  const double chi = c->mp[r].chi;
+  HostNoise &ng = c->noise[r];
+  for (size_t s = 0; s < V; s++)
+    for (int L = 4; L < 10; L++) {
+      u[s * 10 + L] = std::sqrt(chi) * ng.normal();
+      sum[L] += u[s * 10 + L];
+    }
+  const double vol = (double)N * N * N;
+  for (size_t s = 0; s < V; s++)
+    for (int L = 4; L < 10; L++) u[s * 10 + L] -= sum[L] / vol;
+  return lgv_set_field(c, r, u.data());
+}
+
+int lgv_thermalize(lgv_ctx *c, double dt, int nsteps) {
+  if (!c) return fail(LGV_ERR_ARG, "null context");
+  int rc;
+  for (int r = 0; r < c->nrep; r++)
+    if ((rc = lgv_init_gaussian_charges(c, r))) return rc;
+  for (int i = 0; i < nsteps; i++) {
+    const int substeps = 6;
+    for (int j = 0; j < substeps; j++)
+      if ((rc = lgv_hb_phi_step(c, dt / substeps))) return rc;
+  }
+  return LGV_OK;
+}
+
+int lgv_monitor_get(lgv_ctx *c, int r, lgv_monitor out[3]) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  std::vector<int> first(c->nrep);
+  HIPCHECK(hipMemcpy(first.data(), c->plan_half.d_blk_first, c->nrep * sizeof(int),
+                     hipMemcpyDeviceToHost));
+  const int b0 = first[r];
+  const int b1 = r + 1 < c->nrep ? first[r + 1] : c->plan_half.nblocks;
+  std::vector<MonSlot> slots(b1 - b0);
+  MonSlot *src[2] = {c->mon_phi, c->mon_charge};
+  for (int k = 0; k < 2; k++) {
+    HIPCHECK(hipMemcpy(slots.data(), src[k] + b0, slots.size() * sizeof(MonSlot),
+                       hipMemcpyDeviceToHost));
+    lgv_monitor m{};
+    for (const MonSlot &s : slots) {
+      m.down_counts += s.down_counts;
+      m.up_counts += s.up_counts;
+      m.up_yes += s.up_yes;
+      m.up_no += s.up_no;
+      m.down_dS += s.down_dS;
+      m.up_dS += s.up_dS;
+    }
+    out[k] = m;
+  }
+  MonSlot mi;
+  HIPCHECK(hipMemcpy(&mi, c->mon_ideal + r, sizeof mi, hipMemcpyDeviceToHost));
+  out[2] = lgv_monitor{mi.down_counts, mi.up_counts, mi.up_yes, mi.up_no, mi.down_dS, mi.up_dS};
+  return LGV_OK;
+}
+
+int lgv_energy_get(lgv_ctx *c, int r, double out[2]) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  ReplicaState s;
+  HIPCHECK(hipMemcpy(&s, c->dS + r, sizeof s, hipMemcpyDeviceToHost));
+  out[0] = s.E_old;
+  out[1] = s.E_new;
+  return LGV_OK;
+}
+
+int lgv_rng_get_state(lgv_ctx *c, int r, lgv_rng_state *st) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  ReplicaState s;
+  HIPCHECK(hipMemcpy(&s, c->dS + r, sizeof s, hipMemcpyDeviceToHost));
+  st->s0 = c->noise[r].g.s0;
+  st->s1 = c->noise[r].g.s1;
+  st->saved_normal = c->noise[r].saved;
+  st->has_saved_normal = c->noise[r].has_saved;
+  st->sweep_ctr = s.sweep_ctr;
+  return LGV_OK;
+}
+
+int lgv_rng_set_state(lgv_ctx *c, int r, const lgv_rng_state *st) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  c->noise[r].g.s0 = st->s0;
+  c->noise[r].g.s1 = st->s1;
+  c->noise[r].saved = st->saved_normal;
+  c->noise[r].has_saved = st->has_saved_normal;
+  ReplicaState s;
+  HIPCHECK(hipMemcpy(&s, c->dS + r, sizeof s, hipMemcpyDeviceToHost));
+  s.sweep_ctr = st->sweep_ctr;
+  HIPCHECK(hipMemcpy(c->dS + r, &s, sizeof s, hipMemcpyHostToDevice));
+  return LGV_OK;
+}
+
+int lgv_measure(lgv_ctx *c, int r, double *wall, double *wallphase, double *oavg) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  if (!wall || !wallphase || !oavg) return fail(LGV_ERR_ARG, "null output");
+  HIPCHECK(hipSetDevice(c->device));
+  const int N = c->hp[r].N;
+  launch_measure(c->fp32, c->dP, r, N, c->d_partXZ, c->d_partXY, c->d_wall, c->d_wallphase,
+                 c->stream);
+  if ((rc = check_launch())) return rc;
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  HIPCHECK(hipMemcpy(wall, c->d_wall, 3 * 11 * N * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(wallphase, c->d_wallphase, 3 * 14 * N * sizeof(double),
+                     hipMemcpyDeviceToHost));
+  // computeDerivedObs's OAverage (measurer.cxx:288-300): mean of wallX rows
+  for (int l = 0; l < 11; l++) {
+    double s = 0.;
+    for (int i = 0; i < N; i++) s += wall[l * N + i];
+    oavg[l] = s / (double)N;
+  }
+  return LGV_OK;
+}
+
+int lgv_timing_enable(lgv_ctx *c, int enable) {
+  if (!c) return fail(LGV_ERR_ARG, "null context");
+  HIPCHECK(hipSetDevice(c->device));
+  int rc = drain_timing(c);
+  if (rc) return rc;
+  c->timing = enable != 0;
+  for (int k = 0; k < LGV_NCLASS; k++) {
+    c->t_ms[k] = 0.;
+    c->t_launch[k] = 0;
+  }
+  return LGV_OK;
+}
+
+int lgv_timing_get(lgv_ctx *c, double ms[LGV_NCLASS], long long launches[LGV_NCLASS]) {
+  if (!c) return fail(LGV_ERR_ARG, "null context");
+  HIPCHECK(hipSetDevice(c->device));
+  int rc = drain_timing(c);
+  if (rc) return rc;
+  for (int k = 0; k < LGV_NCLASS; k++) {
+    ms[k] = c->t_ms[k];
+    launches[k] = c->t_launch[k];
+  }
+  return LGV_OK;
+}
+
+int lgv_synchronize(lgv_ctx *c) {
+  if (!c) return fail(LGV_ERR_ARG, "null context");
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return LGV_OK;
+}
+
+int lgv_field_device_ptr(lgv_ctx *c, int r, void **ptr, int *elem_bytes) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  *ptr = c->fields[r];
+  *elem_bytes = c->fp32 ? 4 : 8;
+  return LGV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,209 @@
+// lgv_device.h -- device (and host-twin) building blocks of the ModelG
+// kernels: the xoroshiro128+ engine and libstdc++ conversions, per-site
+// counter keys, the red-black structure-of-arrays layout and the O(4)
+// rotation.  Compiled with -ffp-contract=off: every floating-point
+// expression rounds exactly where the reference's does.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define LGV_HD __host__ __device__ __forceinline__
+
+namespace lgv {
+
+// ---------------------------------------------------------------- RNG
+
+struct Xoro {
+  uint64_t s0, s1;
+};
+
+LGV_HD uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+// xoroshiro128plus::seed(uint64_t), ModelG/xoroshiro128plus.h:86-91
+LGV_HD Xoro xoro_seeded(uint64_t seed) {
+  Xoro g;
+  g.s0 = 0x01d353e5f3993bb0ULL + seed;
+  g.s1 = 0x7b9c0df6cb193b20ULL * (seed + 1);
+  return g;
+}
+
+// xoroshiro128plus::operator(), ModelG/xoroshiro128plus.h:97-107
+LGV_HD uint64_t xoro_next(Xoro &g) {
+  const uint64_t s0 = g.s0;
+  uint64_t s1 = g.s1;
+  const uint64_t result = s0 + s1;
+  s1 ^= s0;
+  g.s0 = rotl64(s0, 55) ^ s1 ^ (s1 << 14);
+  g.s1 = rotl64(s1, 36);
+  return result;
+}
+
+// generate_canonical<double,53> over a 64-bit engine (libstdc++ 11
+// bits/random.tcc:3348-3379): double(x) / 2^64, clamped below one.
+// Multiplying by 2^-64 is the same exact scaling as dividing by 2^64.
+LGV_HD double canonical(uint64_t x) {
+  double r = (double)x * 0x1p-64;
+  return r >= 1.0 ? 0x1.fffffffffffffp-1 : r;
+}
+
+// uniform_real_distribution(-sqrt(3), sqrt(3)): u*(b-a)+a, b-a = 2 sqrt(3)
+// (exact doubling); NoiseGenerator::variance1, NoiseGenerator.h:68-69,105
+constexpr double kSqrt3 = 0x1.bb67ae8584caap+0;  // sqrt(3.) rounded
+LGV_HD double variance1(Xoro &g) {
+  const double u = canonical(xoro_next(g));
+  return u * (kSqrt3 + kSqrt3) + (-kSqrt3);
+}
+// uniform_real_distribution(0, 1): u*(1-0)+0 == u; NoiseGenerator::uniform
+LGV_HD double uniform01(Xoro &g) { return canonical(xoro_next(g)); }
+
+// Lemire downscaling of uniform_int_distribution{0,b} with a 64-bit engine,
+// libstdc++ 11 bits/uniform_int_dist.h:246-307
+LGV_HD uint64_t uniform_int(Xoro &g, uint64_t b) {
+  const uint64_t range = b + 1;
+  uint64_t x = xoro_next(g);
+  uint64_t low = x * range;
+#ifdef __HIP_DEVICE_COMPILE__
+  uint64_t high = __umul64hi(x, range);
+#else
+  uint64_t high = (uint64_t)(((unsigned __int128)x * range) >> 64);
+#endif
+  if (low < range) {
+    const uint64_t threshold = (0 - range) % range;
+    while (low < threshold) {
+      x = xoro_next(g);
+      low = x * range;
+#ifdef __HIP_DEVICE_COMPILE__
+      high = __umul64hi(x, range);
+#else
+      high = (uint64_t)(((unsigned __int128)x * range) >> 64);
+#endif
+    }
+  }
+  return high;
+}
+
+// std::shuffle of {0,1,2} then {0,1} (libstdc++ 11 bits/stl_algo.h:
+// 3731-3792, the two-positions-per-draw path); ModelGChargeHB::step,
+// ModelG/Stepper.cxx:461-466.  order[0..2] = directions, order[3..4] = parities.
+LGV_HD void shuffle_orders(Xoro &g, int order[5]) {
+  order[0] = 0; order[1] = 1; order[2] = 2; order[3] = 0; order[4] = 1;
+  {
+    const uint64_t x = uniform_int(g, 2 * 3 - 1);
+    const int p0 = (int)(x / 3), p1 = (int)(x % 3);
+    int t = order[1]; order[1] = order[p0]; order[p0] = t;
+    t = order[2]; order[2] = order[p1]; order[p1] = t;
+  }
+  {
+    const int d = (int)uniform_int(g, 1);
+    int t = order[3 + 1]; order[3 + 1] = order[3 + d]; order[3 + d] = t;
+  }
+}
+
+// Per-site counter streams.  splitmix64's output function (Steele, Lea,
+// Flood 2014) mixes a stream counter and then a site index into a 64-bit
+// key; xoroshiro128plus::seed(key) starts that site's stream.
+LGV_HD uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+LGV_HD uint64_t base_key(uint32_t seed32) {
+  return mix64((uint64_t)seed32 ^ 0x6a09e667f3bcc909ULL);
+}
+LGV_HD uint64_t stream_key(uint64_t base, uint64_t ctr) {
+  return mix64(base + mix64(ctr + 0x9e3779b97f4a7c15ULL));
+}
+LGV_HD uint64_t site_key(uint64_t skey, uint64_t site) {
+  return mix64(skey ^ mix64(site + 0x3c6ef372fe94f82bULL));
+}
+
+// ---------------------------------------------------------------- layout
+//
+// A replica of side N (even) stores component c (0..3 phi, 4..6 A, 7..9 V)
+// of the site (x, y, z) with parity p = (x+y+z)&1 at
+//   base + (c*2 + p) * half + (z*N + y) * (N/2) + x/2,   half = N^3/2.
+// A half-sweep therefore reads one half-lattice and writes the other
+// densely, and x-neighbours sit in the adjacent or the same column.
+
+struct Geo {
+  int N, NH;       // side and half side
+  int64_t half;    // N^3 / 2
+};
+
+LGV_HD int site_x(int xh, int y, int z, int p) { return 2 * xh + ((p + y + z) & 1); }
+
+// ---------------------------------------------------------------- O(4)
+// ModelG/O4AlgebraHelper.h, expression by expression.
+
+LGV_HD void expmul(const double v[3], double w[4]) {
+  const double abs = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  if (abs > 1e-20) {
+    double s, c;
+    s = sin(abs);
+    c = cos(abs);
+    w[0] = c;
+    const double a = -(s / abs);
+    w[1] = a * v[0];
+    w[2] = a * v[1];
+    w[3] = a * v[2];
+  } else {
+    w[0] = 1; w[1] = 0.0; w[2] = 0.0; w[3] = 0.0;
+  }
+}
+
+LGV_HD void quatmul(const double v1[4], const double v2[4], double w[4]) {
+  double dot = 0;
+  dot += v1[1] * v2[1];
+  dot += v1[2] * v2[2];
+  dot += v1[3] * v2[3];
+  w[0] = v1[0] * v2[0] - dot;
+  w[1] = v1[2] * v2[3] - v2[2] * v1[3];
+  w[2] = v1[3] * v2[1] - v1[1] * v2[3];
+  w[3] = v1[1] * v2[2] - v2[1] * v1[2];
+  w[1] += v2[0] * v1[1];
+  w[2] += v2[0] * v1[2];
+  w[3] += v2[0] * v1[3];
+  w[1] += v1[0] * v2[1];
+  w[2] += v1[0] * v2[2];
+  w[3] += v1[0] * v2[3];
+}
+
+// IdealPV2::rotatePhi's per-site work (Stepper.cxx:186-198) with
+// O4Rotation (O4AlgebraHelper.h:78-97): mu = -charge/chi*tau, then
+// phi <- L (phi R).
+LGV_HD void rotate_site(double phi[4], const double A[3], const double V[3],
+                        double chi, double tau) {
+  double vmu[3], amu[3];
+  for (int s = 0; s < 3; s++) vmu[s] = -V[s] / chi * tau;
+  for (int s = 0; s < 3; s++) amu[s] = -A[s] / chi * tau;
+  double tmp1[4], tmp2[4], tmp3[4], out[4];
+  for (int i = 0; i < 3; ++i) {
+    tmp1[i] = -0.5 * (vmu[i] - amu[i]);
+    tmp2[i] = 0.5 * (vmu[i] + amu[i]);
+  }
+  expmul(tmp1, tmp3);
+  expmul(tmp2, tmp1);
+  quatmul(phi, tmp3, tmp2);
+  quatmul(tmp1, tmp2, out);
+  phi[0] = out[0]; phi[1] = out[1]; phi[2] = out[2]; phi[3] = out[3];
+}
+
+// ---------------------------------------------------------------- sums
+// Compensated (TwoSum / Neumaier) accumulation for FP64 reductions.
+struct KSum {
+  double s, c;
+};
+LGV_HD void ksum_add(KSum &k, double x) {
+  const double t = k.s + x;
+  if (fabs(k.s) >= fabs(x)) k.c += (k.s - t) + x;
+  else k.c += (x - t) + k.s;
+  k.s = t;
+}
+LGV_HD void ksum_merge(KSum &a, const KSum &b) {
+  ksum_add(a, b.s);
+  a.c += b.c;
+}
+LGV_HD double ksum_value(const KSum &k) { return k.s + k.c; }
+
+}  // namespace lgv

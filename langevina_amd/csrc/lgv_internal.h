@@ -1,0 +1,81 @@
+// lgv_internal.h -- structures shared by the kernels (lgv_kernels.hip) and
+// the context / C-ABI layer (lgv_api.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lgv {
+
+// Read-only (for the kernels) description of one replica.  The step
+// coefficients (rdtg, rms, dt_ideal, tau) are computed on the host with the
+// reference's expressions for the dt in use.
+struct ReplicaParams {
+  void *field;        // 10 planes x 2 parities x half elements (T = double/float)
+  double *scratch;    // 4 x 2 x half doubles: rotated phi inside IdealPV2
+  int N, NH;
+  int64_t half;       // N^3 / 2
+  double mass0, dmassdt, lambda, H, chi;
+  double axx;         // pow(1/h, 2), Stepper.cxx:91-93
+  double rdtg;        // sqrt(2 dt_hb gamma), Stepper.cxx:301
+  double rms;         // sqrt(2 dt sigma), Stepper.cxx:456
+  double dt_ideal;    // dt / scounts[1]
+  double tau_ideal;   // dt_ideal / 2.0
+  uint64_t base_key;
+};
+
+// Device-written state of one replica (clock, counters, per-step data).
+struct ReplicaState {
+  double t;
+  uint64_t sweep_ctr;
+  int order[5];      // charge sub-sweep order: directions[3], parities[2]
+  int pad;
+  double E_old, E_new;
+};
+
+// Per-block monitor partials (o4_stepper_monitor counters).
+struct MonSlot {
+  long long down_counts, up_counts, up_yes, up_no;
+  double down_dS, up_dS;
+};
+
+struct EnergySlot {
+  double g_s, g_c, h_s, h_c, n_s, n_c;
+};
+
+// A launch over several replicas: block b belongs to replica blk_rep[b] and
+// is its (b - blk_first[replica])-th block.
+struct Plan {
+  int nblocks = 0;
+  int *d_blk_rep = nullptr;
+  int *d_blk_first = nullptr;
+};
+
+constexpr int kBlock = 256;
+
+// Kernel launchers (lgv_kernels.hip).  fp32 selects the storage type.
+void launch_hb_phi(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
+                   int ieo, int ctr_off, MonSlot *mon, hipStream_t st);
+void launch_shuffle(int nrep, ReplicaState *S, const ReplicaParams *P, int ctr_off,
+                    hipStream_t st);
+void launch_charge(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
+                   int sub, int ctr_off, MonSlot *mon, hipStream_t st);
+void launch_ideal_rot1(bool fp32, const Plan &plan, const ReplicaParams *P, hipStream_t st);
+void launch_ideal_kick_rot2(bool fp32, const Plan &plan, const ReplicaParams *P,
+                            hipStream_t st);
+void launch_energy(bool fp32, const Plan &plan, const ReplicaParams *P, EnergySlot *slots,
+                   hipStream_t st);
+void launch_energy_finish(int nrep, const Plan &plan, const ReplicaParams *P,
+                          ReplicaState *S, const EnergySlot *slots, int which,
+                          int ctr_off, MonSlot *mon_ideal, hipStream_t st);
+void launch_advance(int nrep, ReplicaState *S, double dt, int advance_time,
+                    uint64_t nctr, hipStream_t st);
+void launch_set_time(int nrep, int replica, ReplicaState *S, double t, hipStream_t st);
+void launch_aos_to_rb(bool fp32, const ReplicaParams *P, int replica, int N,
+                      const double *aos, hipStream_t st);
+void launch_rb_to_aos(bool fp32, const ReplicaParams *P, int replica, int N, double *aos,
+                      hipStream_t st);
+void launch_measure(bool fp32, const ReplicaParams *P, int replica, int N, double *partXZ,
+                    double *partXY, double *wall, double *wallphase, hipStream_t st);
+
+}  // namespace lgv

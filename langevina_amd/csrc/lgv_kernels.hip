@@ -1,0 +1,622 @@
+// lgv_kernels.hip -- HIP kernels of the ModelG time step for gfx950.
+//
+// One thread per active site, 256-thread blocks (4 wave64s), blocks mapped
+// to replicas through a launch plan, so that one launch steps many
+// independent lattices.  Fields live in the red-black structure-of-arrays
+// layout of lgv_device.h.  All FP arithmetic is FP64 (FP32 storage is
+// widened on load and rounded on store) and follows the reference's
+// expression order; the library is built with -ffp-contract=off.
+#include <float.h>
+
+#include "lgv_device.h"
+#include "lgv_internal.h"
+
+namespace lgv {
+
+template <typename T>
+__device__ __forceinline__ T *plane(const ReplicaParams &p, int c, int par) {
+  return reinterpret_cast<T *>(p.field) + ((int64_t)(c * 2 + par)) * p.half;
+}
+
+__device__ __forceinline__ int wrapp(int a, int N) { return a == N ? 0 : a; }
+__device__ __forceinline__ int wrapm(int a, int N) { return a < 0 ? N - 1 : a; }
+
+// Deterministic block reduction of the monitor increments; thread 0 adds the
+// block total into the block's own slot (stream order makes this race-free).
+__device__ void block_monitor_add(MonSlot *slot, long long down, long long upyes,
+                                  long long upno, double dSd, double dSu) {
+  __shared__ long long sc[3][kBlock / 64];
+  __shared__ double sd[2][kBlock / 64];
+  for (int o = 32; o > 0; o >>= 1) {
+    down += __shfl_xor(down, o, 64);
+    upyes += __shfl_xor(upyes, o, 64);
+    upno += __shfl_xor(upno, o, 64);
+    dSd += __shfl_xor(dSd, o, 64);
+    dSu += __shfl_xor(dSu, o, 64);
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sc[0][w] = down; sc[1][w] = upyes; sc[2][w] = upno;
+    sd[0][w] = dSd; sd[1][w] = dSu;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long a = 0, b = 0, c = 0;
+    double d = 0, e = 0;
+    for (int i = 0; i < kBlock / 64; i++) {
+      a += sc[0][i]; b += sc[1][i]; c += sc[2][i];
+      d += sd[0][i]; e += sd[1][i];
+    }
+    slot->down_counts += a;
+    slot->up_counts += b + c;
+    slot->up_yes += b;
+    slot->up_no += c;
+    slot->down_dS += d;
+    slot->up_dS += e;
+  }
+}
+
+// ------------------------------------------------------------------------
+// One parity of EulerLangevinHB::step (ModelG/Stepper.cxx:309-386).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_hb_phi(const ReplicaParams *__restrict__ P,
+                                                   const ReplicaState *__restrict__ S,
+                                                   const int *__restrict__ blk_rep,
+                                                   const int *__restrict__ blk_first, int ieo,
+                                                   int ctr_off, MonSlot *__restrict__ mon) {
+  const int r = blk_rep[blockIdx.x];
+  const ReplicaParams p = P[r];
+  const int64_t idx = (int64_t)(blockIdx.x - blk_first[r]) * kBlock + threadIdx.x;
+  long long down = 0, upyes = 0, upno = 0;
+  double dSd = 0., dSu = 0.;
+  if (idx < p.half) {
+    const int N = p.N, NH = p.NH;
+    const int xh = (int)(idx % NH);
+    const int64_t rowi = idx / NH;
+    const int y = (int)(rowi % N), z = (int)(rowi / N);
+    const int x = site_x(xh, y, z, ieo);
+    const int q = ieo ^ 1;
+    const int64_t row = rowi * NH;
+    const int64_t nb[6] = {row + (wrapp(x + 1, N) >> 1), row + (wrapm(x - 1, N) >> 1),
+                           ((int64_t)z * N + wrapp(y + 1, N)) * NH + xh,
+                           ((int64_t)z * N + wrapm(y - 1, N)) * NH + xh,
+                           ((int64_t)wrapp(z + 1, N) * N + y) * NH + xh,
+                           ((int64_t)wrapm(z - 1, N) * N + y) * NH + xh};
+    const ReplicaState s = S[r];
+    const double mass = p.mass0 + p.dmassdt * s.t;
+    const double Lambda2 = 0.5 * (2. * 3. + mass);
+    const double lambda = p.lambda;
+    const double H = p.H;
+    const double rdtg = p.rdtg;
+    Xoro g = xoro_seeded(site_key(stream_key(p.base_key, s.sweep_ctr + ctr_off),
+                                  ((uint64_t)z * N + y) * N + x));
+    double hphi_o = 0., hphi_n = 0., s_o = 0., s_n = 0.;
+    double phi_o[4], phi_n[4];
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+      const T *fq = plane<T>(p, l, q);
+      const double heff = ((double)fq[nb[0]] + (double)fq[nb[1]]) +
+                          ((double)fq[nb[2]] + (double)fq[nb[3]]) +
+                          ((double)fq[nb[4]] + (double)fq[nb[5]]);
+      phi_o[l] = (double)plane<T>(p, l, ieo)[idx];
+      phi_n[l] = phi_o[l] + rdtg * variance1(g);
+      s_o += phi_o[l] * phi_o[l];
+      s_n += phi_n[l] * phi_n[l];
+      hphi_o += heff * phi_o[l];
+      hphi_n += heff * phi_n[l];
+    }
+    hphi_o += (H * phi_o[0]);
+    hphi_n += (H * phi_n[0]);
+    const double dS = -(hphi_n - hphi_o) + Lambda2 * (s_n - s_o) +
+                      lambda / 4. * (s_n * s_n - s_o * s_o);
+    bool accept;
+    if (dS < 0) {
+      accept = true;
+      down = 1;
+      dSd = dS;
+    } else {
+      const double rr = uniform01(g);
+      accept = rr < exp(-dS);
+      if (accept) { upyes = 1; dSu = dS; } else { upno = 1; }
+    }
+    if (accept) {
+#pragma unroll
+      for (int l = 0; l < 4; l++) plane<T>(p, l, ieo)[idx] = (T)phi_n[l];
+    }
+  }
+  block_monitor_add(mon + blockIdx.x, down, upyes, upno, dSd, dSu);
+}
+
+// ------------------------------------------------------------------------
+// The shuffled sub-sweep order of ModelGChargeHB::step (Stepper.cxx:461-468)
+// from the replica's per-step shuffle stream.
+__global__ void k_shuffle(ReplicaState *S, const ReplicaParams *P, int nrep, int ctr_off) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrep) return;
+  Xoro g = xoro_seeded(site_key(stream_key(P[r].base_key, S[r].sweep_ctr + ctr_off), 0));
+  int order[5];
+  shuffle_orders(g, order);
+  for (int i = 0; i < 5; i++) S[r].order[i] = order[i];
+}
+
+// ------------------------------------------------------------------------
+// One (direction, parity) sub-sweep of ModelGChargeHB::step
+// (Stepper.cxx:474-518 with modelg_update_charge_pair :419-441).  Pairs
+// (A, A+e_dir) are disjoint, so each thread owns its pair.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_charge(const ReplicaParams *__restrict__ P,
+                                                   const ReplicaState *__restrict__ S,
+                                                   const int *__restrict__ blk_rep,
+                                                   const int *__restrict__ blk_first, int sub,
+                                                   int ctr_off, MonSlot *__restrict__ mon) {
+  const int r = blk_rep[blockIdx.x];
+  const ReplicaParams p = P[r];
+  const int64_t idx = (int64_t)(blockIdx.x - blk_first[r]) * kBlock + threadIdx.x;
+  long long down = 0, upyes = 0, upno = 0;
+  double dSd = 0., dSu = 0.;
+  if (idx < p.half) {
+    const ReplicaState s = S[r];
+    const int dir = s.order[sub >> 1];
+    const int eoA = s.order[3 + (sub & 1)];
+    const int N = p.N, NH = p.NH;
+    const int xh = (int)(idx % NH);
+    const int64_t rowi = idx / NH;
+    const int y = (int)(rowi % N), z = (int)(rowi / N);
+    const int x = site_x(xh, y, z, eoA);
+    int64_t idxB;
+    if (dir == 0) idxB = rowi * NH + (wrapp(x + 1, N) >> 1);
+    else if (dir == 1) idxB = ((int64_t)z * N + wrapp(y + 1, N)) * NH + xh;
+    else idxB = ((int64_t)wrapp(z + 1, N) * N + y) * NH + xh;
+    const double chi = p.chi, rms = p.rms;
+    Xoro g = xoro_seeded(site_key(stream_key(p.base_key, s.sweep_ctr + ctr_off),
+                                  ((uint64_t)z * N + y) * N + x));
+#pragma unroll
+    for (int L = 4; L < 10; L++) {
+      T *pa = plane<T>(p, L, eoA);
+      T *pb = plane<T>(p, L, eoA ^ 1);
+      const double nA = (double)pa[idx], nB = (double)pb[idxB];
+      double q = rms * variance1(g);
+      const double dS = ((nA - q) * (nA - q) + (nB + q) * (nB + q) - nA * nA - nB * nB) /
+                        (2. * chi);
+      if (dS < 0) {
+        down++;
+        dSd += dS;
+      } else {
+        const double rr = uniform01(g);
+        if (rr < exp(-dS)) {
+          upyes++;
+          dSu += dS;
+        } else {
+          upno++;
+          q = 0.;
+        }
+      }
+      pa[idx] = (T)(nA + (-q));
+      pb[idxB] = (T)(nB + q);
+    }
+  }
+  block_monitor_add(mon + blockIdx.x, down, upyes, upno, dSd, dSu);
+}
+
+// ------------------------------------------------------------------------
+// IdealPV2::step_no_reject (Stepper.cxx:68-176), split in two passes:
+// rot1 rotates every site by dt/2 into an FP64 scratch copy of phi,
+// kick_rot2 kicks the charges with the rotated neighbours and rotates again.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_ideal_rot1(const ReplicaParams *__restrict__ P,
+                                                       const int *__restrict__ blk_rep,
+                                                       const int *__restrict__ blk_first) {
+  const int r = blk_rep[blockIdx.x];
+  const ReplicaParams p = P[r];
+  int64_t idx = (int64_t)(blockIdx.x - blk_first[r]) * kBlock + threadIdx.x;
+  if (idx >= 2 * p.half) return;
+  const int par = idx >= p.half;
+  idx -= par * p.half;
+  double phi[4], A[3], V[3];
+  for (int l = 0; l < 4; l++) phi[l] = (double)plane<T>(p, l, par)[idx];
+  for (int l = 0; l < 3; l++) A[l] = (double)plane<T>(p, 4 + l, par)[idx];
+  for (int l = 0; l < 3; l++) V[l] = (double)plane<T>(p, 7 + l, par)[idx];
+  rotate_site(phi, A, V, p.chi, p.tau_ideal);
+  for (int l = 0; l < 4; l++) p.scratch[(int64_t)(l * 2 + par) * p.half + idx] = phi[l];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_ideal_kick_rot2(
+    const ReplicaParams *__restrict__ P, const int *__restrict__ blk_rep,
+    const int *__restrict__ blk_first) {
+  const int r = blk_rep[blockIdx.x];
+  const ReplicaParams p = P[r];
+  int64_t idx = (int64_t)(blockIdx.x - blk_first[r]) * kBlock + threadIdx.x;
+  if (idx >= 2 * p.half) return;
+  const int par = idx >= p.half;
+  idx -= par * p.half;
+  const int N = p.N, NH = p.NH;
+  const int xh = (int)(idx % NH);
+  const int64_t rowi = idx / NH;
+  const int y = (int)(rowi % N), z = (int)(rowi / N);
+  const int x = site_x(xh, y, z, par);
+  const int q = par ^ 1;
+  const int64_t row = rowi * NH;
+  const int64_t ixp = row + (wrapp(x + 1, N) >> 1), ixm = row + (wrapm(x - 1, N) >> 1);
+  const int64_t iyp = ((int64_t)z * N + wrapp(y + 1, N)) * NH + xh;
+  const int64_t iym = ((int64_t)z * N + wrapm(y - 1, N)) * NH + xh;
+  const int64_t izp = ((int64_t)wrapp(z + 1, N) * N + y) * NH + xh;
+  const int64_t izm = ((int64_t)wrapm(z - 1, N) * N + y) * NH + xh;
+  const double *sc = p.scratch;
+  double cp[4], xp[4], xm[4], yp[4], ym[4], zp[4], zm[4];
+  for (int l = 0; l < 4; l++) {
+    const double *own = sc + (int64_t)(l * 2 + par) * p.half;
+    const double *oth = sc + (int64_t)(l * 2 + q) * p.half;
+    cp[l] = own[idx];
+    xp[l] = oth[ixp]; xm[l] = oth[ixm];
+    yp[l] = oth[iyp]; ym[l] = oth[iym];
+    zp[l] = oth[izp]; zm[l] = oth[izm];
+  }
+  double A[3], V[3];
+  for (int l = 0; l < 3; l++) A[l] = (double)plane<T>(p, 4 + l, par)[idx];
+  for (int l = 0; l < 3; l++) V[l] = (double)plane<T>(p, 7 + l, par)[idx];
+  const double axx = p.axx, ayy = p.axx, azz = p.axx;
+  const double H0 = p.H, dt = p.dt_ideal;
+  for (int l = 0; l < 3; l++) {
+    const double advxx = (-xp[0] * cp[l + 1] + cp[0] * xp[l + 1] + cp[0] * xm[l + 1] -
+                          xm[0] * cp[l + 1]) * axx;
+    const double advyy = (-yp[0] * cp[l + 1] + cp[0] * yp[l + 1] + cp[0] * ym[l + 1] -
+                          ym[0] * cp[l + 1]) * ayy;
+    const double advzz = (-zp[0] * cp[l + 1] + cp[0] * zp[l + 1] + cp[0] * zm[l + 1] -
+                          zm[0] * cp[l + 1]) * azz;
+    A[l] += dt * (advxx + advyy + advzz - H0 * cp[l + 1]);
+  }
+  for (int s = 0; s < 3; s++) {
+    const int s1 = (s + 1) % 3, s2 = (s + 2) % 3;
+    const double advxx = (xp[1 + s2] * cp[1 + s1] - cp[1 + s2] * xm[1 + s1] -
+                          xp[1 + s1] * cp[1 + s2] + cp[1 + s1] * xm[1 + s2]) * axx;
+    const double advyy = (yp[1 + s2] * cp[1 + s1] - cp[1 + s2] * ym[1 + s1] -
+                          yp[1 + s1] * cp[1 + s2] + cp[1 + s1] * ym[1 + s2]) * ayy;
+    const double advzz = (zp[1 + s2] * cp[1 + s1] - cp[1 + s2] * zm[1 + s1] -
+                          zp[1 + s1] * cp[1 + s2] + cp[1 + s1] * zm[1 + s2]) * azz;
+    V[s] += dt * (advxx + advyy + advzz);
+  }
+  rotate_site(cp, A, V, p.chi, p.tau_ideal);
+  for (int l = 0; l < 4; l++) plane<T>(p, l, par)[idx] = (T)cp[l];
+  for (int l = 0; l < 3; l++) plane<T>(p, 4 + l, par)[idx] = (T)A[l];
+  for (int l = 0; l < 3; l++) plane<T>(p, 7 + l, par)[idx] = (T)V[l];
+}
+
+// ------------------------------------------------------------------------
+// IdealPV2::computeEnergy (Stepper.cxx:204-268): per-block compensated
+// partial sums of grad2, hEn and nab2.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_energy(const ReplicaParams *__restrict__ P,
+                                                   const int *__restrict__ blk_rep,
+                                                   const int *__restrict__ blk_first,
+                                                   EnergySlot *__restrict__ slots) {
+  const int r = blk_rep[blockIdx.x];
+  const ReplicaParams p = P[r];
+  int64_t idx = (int64_t)(blockIdx.x - blk_first[r]) * kBlock + threadIdx.x;
+  double g2 = 0., hE = 0., n2 = 0.;
+  if (idx < 2 * p.half) {
+    const int par = idx >= p.half;
+    idx -= par * p.half;
+    const int N = p.N, NH = p.NH;
+    const int xh = (int)(idx % NH);
+    const int64_t rowi = idx / NH;
+    const int y = (int)(rowi % N), z = (int)(rowi / N);
+    const int x = site_x(xh, y, z, par);
+    const int q = par ^ 1;
+    const int64_t ixp = rowi * NH + (wrapp(x + 1, N) >> 1);
+    const int64_t iyp = ((int64_t)z * N + wrapp(y + 1, N)) * NH + xh;
+    const int64_t izp = ((int64_t)wrapp(z + 1, N) * N + y) * NH + xh;
+    for (int s = 0; s < 4; s++) {
+      const double phimid = (double)plane<T>(p, s, par)[idx];
+      const T *o = plane<T>(p, s, q);
+      double d;
+      d = (double)o[izp] - phimid; g2 += d * d;
+      d = (double)o[iyp] - phimid; g2 += d * d;
+      d = (double)o[ixp] - phimid; g2 += d * d;
+      if (s == 0) hE = phimid * p.H;
+    }
+    for (int s = 0; s < 3; s++) { const double v = (double)plane<T>(p, 7 + s, par)[idx]; n2 += v * v; }
+    for (int s = 0; s < 3; s++) { const double a = (double)plane<T>(p, 4 + s, par)[idx]; n2 += a * a; }
+  }
+  KSum kg{g2, 0.}, kh{hE, 0.}, kn{n2, 0.};
+  for (int o = 32; o > 0; o >>= 1) {
+    KSum t;
+    t.s = __shfl_xor(kg.s, o, 64); t.c = __shfl_xor(kg.c, o, 64); ksum_merge(kg, t);
+    t.s = __shfl_xor(kh.s, o, 64); t.c = __shfl_xor(kh.c, o, 64); ksum_merge(kh, t);
+    t.s = __shfl_xor(kn.s, o, 64); t.c = __shfl_xor(kn.c, o, 64); ksum_merge(kn, t);
+  }
+  __shared__ KSum sh[3][kBlock / 64];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { sh[0][w] = kg; sh[1][w] = kh; sh[2][w] = kn; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    KSum a = sh[0][0], b = sh[1][0], c = sh[2][0];
+    for (int i = 1; i < kBlock / 64; i++) {
+      ksum_merge(a, sh[0][i]); ksum_merge(b, sh[1][i]); ksum_merge(c, sh[2][i]);
+    }
+    slots[blockIdx.x] = EnergySlot{a.s, a.c, b.s, b.c, c.s, c.c};
+  }
+}
+
+// Combines a replica's energy partials; which = 0 stores E_old, 1 stores
+// E_new and applies IdealPV2::step's statistics (Stepper.cxx:37-55): if
+// deltaE > 0 one uniform is drawn and compared with exp(-deltaE).
+__global__ void k_energy_finish(const ReplicaParams *P, ReplicaState *S,
+                                const int *blk_first, int nrep, int nblocks,
+                                const EnergySlot *slots, int which, int ctr_off,
+                                MonSlot *mon_ideal) {
+  const int r = blockIdx.x;
+  if (r >= nrep) return;
+  const int b0 = blk_first[r];
+  const int b1 = (r + 1 < nrep) ? blk_first[r + 1] : nblocks;
+  KSum g{0, 0}, h{0, 0}, n{0, 0};
+  // lanes stride over the replica's blocks, then a fixed-order merge
+  for (int b = b0 + (int)threadIdx.x; b < b1; b += 64) {
+    ksum_merge(g, KSum{slots[b].g_s, slots[b].g_c});
+    ksum_merge(h, KSum{slots[b].h_s, slots[b].h_c});
+    ksum_merge(n, KSum{slots[b].n_s, slots[b].n_c});
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    KSum t;
+    t.s = __shfl_xor(g.s, o, 64); t.c = __shfl_xor(g.c, o, 64); ksum_merge(g, t);
+    t.s = __shfl_xor(h.s, o, 64); t.c = __shfl_xor(h.c, o, 64); ksum_merge(h, t);
+    t.s = __shfl_xor(n.s, o, 64); t.c = __shfl_xor(n.c, o, 64); ksum_merge(n, t);
+  }
+  if (threadIdx.x != 0) return;
+  const ReplicaParams &p = P[r];
+  double e = 0;
+  e += 0.5 / p.chi * ksum_value(n);
+  e += 0.5 * ksum_value(g);
+  e -= ksum_value(h);
+  if (which == 0) {
+    S[r].E_old = e;
+    return;
+  }
+  S[r].E_new = e;
+  const double deltaE = e - S[r].E_old;
+  MonSlot &m = mon_ideal[r];
+  if (deltaE > 0) {
+    Xoro gg = xoro_seeded(site_key(stream_key(p.base_key, S[r].sweep_ctr + ctr_off), 0));
+    const double prob = uniform01(gg);
+    if (prob > exp(-deltaE)) { m.up_counts++; m.up_no++; }
+    else { m.up_counts++; m.up_yes++; m.up_dS += deltaE; }
+  } else {
+    m.down_counts++;
+    m.down_dS += deltaE;
+  }
+}
+
+// ------------------------------------------------------------------------
+__global__ void k_advance(ReplicaState *S, int nrep, double dt, int advance_time,
+                          uint64_t nctr) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrep) return;
+  if (advance_time) S[r].t += dt;
+  S[r].sweep_ctr += nctr;
+}
+
+__global__ void k_set_time(ReplicaState *S, int nrep, int replica, double t) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrep) return;
+  if (replica < 0 || r == replica) S[r].t = t;
+}
+
+// ------------------------------------------------------------------------
+// o4fields [z][y][x][10] <-> red-black planes.
+template <typename T>
+__global__ void k_aos_to_rb(const ReplicaParams *P, int replica, const double *aos) {
+  const ReplicaParams p = P[replica];
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t V = 2 * p.half;
+  if (s >= V) return;
+  const int N = p.N;
+  const int x = (int)(s % N);
+  const int64_t yz = s / N;
+  const int y = (int)(yz % N), z = (int)(yz / N);
+  const int par = (x + y + z) & 1;
+  const int64_t i = yz * p.NH + (x >> 1);
+  for (int c = 0; c < 10; c++) plane<T>(p, c, par)[i] = (T)aos[s * 10 + c];
+}
+
+template <typename T>
+__global__ void k_rb_to_aos(const ReplicaParams *P, int replica, double *aos) {
+  const ReplicaParams p = P[replica];
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t V = 2 * p.half;
+  if (s >= V) return;
+  const int N = p.N;
+  const int x = (int)(s % N);
+  const int64_t yz = s / N;
+  const int y = (int)(yz % N), z = (int)(yz / N);
+  const int par = (x + y + z) & 1;
+  const int64_t i = yz * p.NH + (x >> 1);
+  for (int c = 0; c < 10; c++) aos[s * 10 + c] = (double)plane<T>(p, c, par)[i];
+}
+
+// ------------------------------------------------------------------------
+// Measurer observables of one site (measurer.cxx:245-269 and :351-392):
+// o[0..10] = phi, A, V, phi^2; o[11..24] = |phi|, n, A', V', |phi|^2.
+__device__ __forceinline__ void contract_rho(const double n[4], const double rho[6],
+                                             double out[4]) {
+  out[0] = -(n[1] * rho[0]) - n[2] * rho[1] - n[3] * rho[2];
+  out[1] = n[0] * rho[0] + n[3] * rho[4] - n[2] * rho[5];
+  out[2] = n[0] * rho[1] - n[3] * rho[3] + n[1] * rho[5];
+  out[3] = n[0] * rho[2] + n[2] * rho[3] - n[1] * rho[4];
+}
+
+template <typename T>
+__device__ __forceinline__ void site_obs(const ReplicaParams &p, int par, int64_t i,
+                                         double o[25]) {
+  double f[10];
+  for (int c = 0; c < 10; c++) f[c] = (double)plane<T>(p, c, par)[i];
+  double phi2 = 0.0;
+  for (int l = 0; l < 4; l++) phi2 += f[l] * f[l];
+  for (int c = 0; c < 10; c++) o[c] = f[c];
+  o[10] = phi2;
+  const double sigma = sqrt(phi2);
+  double n[4];
+  if (phi2 > 100. * DBL_MIN) {
+    for (int l = 0; l < 4; l++) n[l] = f[l] / sigma;
+  } else {
+    n[0] = 1; n[1] = 0; n[2] = 0; n[3] = 0;
+  }
+  const double rho[6] = {f[4], f[5], f[6], f[7], f[8], f[9]};
+  const double drho[6] = {f[7], f[8], f[9], f[4], f[5], f[6]};
+  double A[4], V[4];
+  contract_rho(n, rho, A);
+  contract_rho(n, drho, V);
+  o[11] = sigma;
+  for (int l = 0; l < 4; l++) {
+    o[12 + l] = n[l];
+    o[16 + l] = A[l];
+    o[20 + l] = V[l];
+  }
+  o[24] = sigma * sigma;
+}
+
+// One thread per (x, z): compensated sum over y -> partXZ[o][z][x];
+// one thread per (x, y): compensated sum over z -> partXY[o][y][x].
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_measure_cols(const ReplicaParams *P, int replica,
+                                                         int along_z, double *part) {
+  const ReplicaParams p = P[replica];
+  const int N = p.N;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)N * N) return;
+  const int x = (int)(t % N), w = (int)(t / N);
+  KSum acc[25];
+  for (int o = 0; o < 25; o++) acc[o] = KSum{0., 0.};
+  for (int v = 0; v < N; v++) {
+    const int y = along_z ? w : v;
+    const int z = along_z ? v : w;
+    const int par = (x + y + z) & 1;
+    const int64_t i = ((int64_t)z * N + y) * p.NH + (x >> 1);
+    double o[25];
+    site_obs<T>(p, par, i, o);
+    for (int k = 0; k < 25; k++) ksum_add(acc[k], o[k]);
+  }
+  for (int k = 0; k < 25; k++) part[((int64_t)k * N + w) * N + x] = ksum_value(acc[k]);
+}
+
+// wall[d][obs][n]: d = 0 (x) sums partXZ over z, d = 2 (z) sums partXZ over
+// x, d = 1 (y) sums partXY over x; then / N^2 as measurer.cxx:274-280.
+__global__ void k_measure_reduce(int N, const double *partXZ, const double *partXY,
+                                 double *wall, double *wallphase) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * 25 * N) return;
+  const int n = t % N;
+  const int k = (t / N) % 25;
+  const int d = t / (25 * N);
+  KSum acc{0., 0.};
+  for (int m = 0; m < N; m++) {
+    double v;
+    if (d == 0) v = partXZ[((int64_t)k * N + m) * N + n];
+    else if (d == 2) v = partXZ[((int64_t)k * N + n) * N + m];
+    else v = partXY[((int64_t)k * N + n) * N + m];
+    ksum_add(acc, v);
+  }
+  const double val = ksum_value(acc) / (double)(N * N);
+  if (k < 11) wall[((int64_t)d * 11 + k) * N + n] = val;
+  else wallphase[((int64_t)d * 14 + (k - 11)) * N + n] = val;
+}
+
+// ======================================================================== launchers
+
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+void launch_hb_phi(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
+                   int ieo, int ctr_off, MonSlot *mon, hipStream_t st) {
+  if (fp32)
+    k_hb_phi<float><<<plan.nblocks, kBlock, 0, st>>>(P, S, plan.d_blk_rep, plan.d_blk_first,
+                                                     ieo, ctr_off, mon);
+  else
+    k_hb_phi<double><<<plan.nblocks, kBlock, 0, st>>>(P, S, plan.d_blk_rep, plan.d_blk_first,
+                                                      ieo, ctr_off, mon);
+}
+
+void launch_shuffle(int nrep, ReplicaState *S, const ReplicaParams *P, int ctr_off,
+                    hipStream_t st) {
+  k_shuffle<<<nblk(nrep, 64), 64, 0, st>>>(S, P, nrep, ctr_off);
+}
+
+void launch_charge(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
+                   int sub, int ctr_off, MonSlot *mon, hipStream_t st) {
+  if (fp32)
+    k_charge<float><<<plan.nblocks, kBlock, 0, st>>>(P, S, plan.d_blk_rep, plan.d_blk_first,
+                                                     sub, ctr_off, mon);
+  else
+    k_charge<double><<<plan.nblocks, kBlock, 0, st>>>(P, S, plan.d_blk_rep, plan.d_blk_first,
+                                                      sub, ctr_off, mon);
+}
+
+void launch_ideal_rot1(bool fp32, const Plan &plan, const ReplicaParams *P, hipStream_t st) {
+  if (fp32)
+    k_ideal_rot1<float><<<plan.nblocks, kBlock, 0, st>>>(P, plan.d_blk_rep, plan.d_blk_first);
+  else
+    k_ideal_rot1<double><<<plan.nblocks, kBlock, 0, st>>>(P, plan.d_blk_rep, plan.d_blk_first);
+}
+
+void launch_ideal_kick_rot2(bool fp32, const Plan &plan, const ReplicaParams *P,
+                            hipStream_t st) {
+  if (fp32)
+    k_ideal_kick_rot2<float><<<plan.nblocks, kBlock, 0, st>>>(P, plan.d_blk_rep,
+                                                              plan.d_blk_first);
+  else
+    k_ideal_kick_rot2<double><<<plan.nblocks, kBlock, 0, st>>>(P, plan.d_blk_rep,
+                                                               plan.d_blk_first);
+}
+
+void launch_energy(bool fp32, const Plan &plan, const ReplicaParams *P, EnergySlot *slots,
+                   hipStream_t st) {
+  if (fp32)
+    k_energy<float><<<plan.nblocks, kBlock, 0, st>>>(P, plan.d_blk_rep, plan.d_blk_first,
+                                                     slots);
+  else
+    k_energy<double><<<plan.nblocks, kBlock, 0, st>>>(P, plan.d_blk_rep, plan.d_blk_first,
+                                                      slots);
+}
+
+void launch_energy_finish(int nrep, const Plan &plan, const ReplicaParams *P,
+                          ReplicaState *S, const EnergySlot *slots, int which,
+                          int ctr_off, MonSlot *mon_ideal, hipStream_t st) {
+  k_energy_finish<<<nrep, 64, 0, st>>>(P, S, plan.d_blk_first, nrep, plan.nblocks, slots,
+                                       which, ctr_off, mon_ideal);
+}
+
+void launch_advance(int nrep, ReplicaState *S, double dt, int advance_time, uint64_t nctr,
+                    hipStream_t st) {
+  k_advance<<<nblk(nrep, 64), 64, 0, st>>>(S, nrep, dt, advance_time, nctr);
+}
+
+void launch_set_time(int nrep, int replica, ReplicaState *S, double t, hipStream_t st) {
+  k_set_time<<<nblk(nrep, 64), 64, 0, st>>>(S, nrep, replica, t);
+}
+
+void launch_aos_to_rb(bool fp32, const ReplicaParams *P, int replica, int N,
+                      const double *aos, hipStream_t st) {
+  const int64_t V = (int64_t)N * N * N;
+  if (fp32) k_aos_to_rb<float><<<nblk(V, 256), 256, 0, st>>>(P, replica, aos);
+  else k_aos_to_rb<double><<<nblk(V, 256), 256, 0, st>>>(P, replica, aos);
+}
+
+void launch_rb_to_aos(bool fp32, const ReplicaParams *P, int replica, int N, double *aos,
+                      hipStream_t st) {
+  const int64_t V = (int64_t)N * N * N;
+  if (fp32) k_rb_to_aos<float><<<nblk(V, 256), 256, 0, st>>>(P, replica, aos);
+  else k_rb_to_aos<double><<<nblk(V, 256), 256, 0, st>>>(P, replica, aos);
+}
+
+void launch_measure(bool fp32, const ReplicaParams *P, int replica, int N, double *partXZ,
+                    double *partXY, double *wall, double *wallphase, hipStream_t st) {
+  const int64_t cols = (int64_t)N * N;
+  if (fp32) {
+    k_measure_cols<float><<<nblk(cols, kBlock), kBlock, 0, st>>>(P, replica, 0, partXZ);
+    k_measure_cols<float><<<nblk(cols, kBlock), kBlock, 0, st>>>(P, replica, 1, partXY);
+  } else {
+    k_measure_cols<double><<<nblk(cols, kBlock), kBlock, 0, st>>>(P, replica, 0, partXZ);
+    k_measure_cols<double><<<nblk(cols, kBlock), kBlock, 0, st>>>(P, replica, 1, partXY);
+  }
+  k_measure_reduce<<<nblk(3 * 25 * N, 256), 256, 0, st>>>(N, partXZ, partXY, wall, wallphase);
+}
+
+}  // namespace lgv

@@ -1,0 +1,208 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same seeded inputs.
+
+Bars: the phi and charge Metropolis sweeps (random streams, proposals,
+accept/reject decisions, stored fields, monitor counts) are bit-exact; one
+IdealPV2 substep and one full PV2HBSplit23 step agree to 1e-12 relative
+(device sin/cos differ from glibc by an ulp); FP32 storage matches the
+oracle's FP32 mode to one float ulp.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+PAR = dict(mass0=-4.813, lam=1.0, H=0.01, chi=2.0, gamma=1.0)
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def L():
+    import langevina_amd
+    langevina_amd.lib()
+    return langevina_amd
+
+
+def make_pair(L, orc, N, seed=42, precision=0, scale=1.0, state_seed=0, **kw):
+    par = dict(PAR)
+    par.update(kw)
+    ctx = L.Context(L.model_params(N, seed=seed, **par), precision=precision)
+    lat = orc.Lattice(N, mass0=par["mass0"], lam=par["lam"], H=par["H"], chi=par["chi"],
+                      gamma=par["gamma"], seed=seed, rng_mode=orc.RNG_PER_SITE,
+                      precision=precision)
+    rng = np.random.default_rng(state_seed)
+    u = rng.normal(size=(N, N, N, 10)) * scale
+    if precision == 1:
+        u = u.astype(np.float32).astype(np.float64)
+    lat.u[...] = u
+    ctx.set_field(u)
+    return ctx, lat
+
+
+def rel_err(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("N", [8, 16, 32])
+def test_field_roundtrip(L, orc, N):
+    log("roundtrip", N)
+    ctx = L.Context(L.model_params(N))
+    u = np.random.default_rng(N).normal(size=(N, N, N, 10))
+    ctx.set_field(u)
+    np.testing.assert_array_equal(ctx.get_field(), u)
+    ctx.close()
+
+
+@pytest.mark.parametrize("N,prec", [(8, 0), (16, 0), (16, 1), (32, 0)])
+def test_hb_phi_sweep_bit_exact(L, orc, N, prec):
+    log("hb", N, prec)
+    ctx, lat = make_pair(L, orc, N, precision=prec, scale=0.7)
+    for dt in (0.04 / 6, 0.3):
+        ctx.hb_phi_step(dt)
+        lat.hb_phi_step(dt)
+        np.testing.assert_array_equal(ctx.get_field(), lat.u)
+    m_dev, m_orc = ctx.monitors()["phi"], lat.monitors()["phi"]
+    for k in ("down_counts", "up_counts", "up_yes", "up_no"):
+        assert m_dev[k] == m_orc[k], k
+    for k in ("down_dS", "up_dS"):
+        assert abs(m_dev[k] - m_orc[k]) <= 1e-9 * abs(m_orc[k]) + 1e-12
+    ctx.close()
+
+
+@pytest.mark.parametrize("N,prec", [(8, 0), (16, 0), (16, 1), (32, 0)])
+def test_charge_sweep_bit_exact(L, orc, N, prec):
+    log("charge", N, prec)
+    ctx, lat = make_pair(L, orc, N, precision=prec, seed=7)
+    for dt in (0.04, 0.5):
+        ctx.charge_step(dt)
+        lat.charge_step(dt)
+        np.testing.assert_array_equal(ctx.get_field(), lat.u)
+    m_dev, m_orc = ctx.monitors()["charge"], lat.monitors()["charge"]
+    for k in ("down_counts", "up_counts", "up_yes", "up_no"):
+        assert m_dev[k] == m_orc[k], k
+    ctx.close()
+
+
+@pytest.mark.parametrize("N,prec", [(8, 0), (16, 0), (16, 1)])
+def test_ideal_step(L, orc, N, prec):
+    log("ideal", N, prec)
+    ctx, lat = make_pair(L, orc, N, precision=prec, seed=3, scale=0.8)
+    ctx.ideal_step(0.04 / 3)
+    lat.ideal_step(0.04 / 3)
+    got = ctx.get_field()
+    if prec == 0:
+        assert rel_err(got, lat.u) <= 1e-12
+    else:
+        # FP32 storage: at most one float ulp apart
+        ulp = np.spacing(np.abs(lat.u).astype(np.float32)).astype(np.float64)
+        assert (np.abs(got - lat.u) <= ulp).all()
+    ctx.close()
+
+
+@pytest.mark.parametrize("N", [8, 16, 32])
+def test_full_step_fp64(L, orc, N):
+    log("full step", N)
+    ctx, lat = make_pair(L, orc, N, seed=11, scale=0.6)
+    for _ in range(2):
+        ctx.step(0.04)
+        lat.step(0.04)
+    assert rel_err(ctx.get_field(), lat.u) <= 1e-12
+    assert ctx.rng_state().sweep_ctr == lat.c.sweep_ctr
+    m_dev, m_orc = ctx.monitors(), lat.monitors()
+    for kind in ("phi", "charge"):
+        assert m_dev[kind]["up_counts"] + m_dev[kind]["down_counts"] == \
+            m_orc[kind]["up_counts"] + m_orc[kind]["down_counts"]
+    ctx.close()
+
+
+def test_thermalize_and_run_with_clock(L, orc):
+    log("thermalize")
+    N = 16
+    ctx = L.Context(L.model_params(N, seed=42, dmassdt=0.5, **PAR))
+    lat = orc.Lattice(N, seed=42, dmassdt=0.5, rng_mode=orc.RNG_PER_SITE, **{
+        "mass0": PAR["mass0"], "lam": PAR["lam"], "H": PAR["H"], "chi": PAR["chi"]})
+    ctx.thermalize(0.04, 3)
+    lat.thermalize(0.04, 3)
+    np.testing.assert_array_equal(ctx.get_field(), lat.u)
+    ctx.run(0.04, 3)
+    for _ in range(3):
+        lat.step(0.04)
+        lat.t = lat.t + 0.04
+    assert ctx.time() == lat.t
+    assert rel_err(ctx.get_field(), lat.u) <= 1e-12
+    ctx.close()
+
+
+def test_graph_replay_matches_stream(L, orc):
+    log("graph")
+    N = 16
+    a, _ = make_pair(L, orc, N, seed=5)
+    b, _ = make_pair(L, orc, N, seed=5)
+    b.use_graph(True)
+    a.run(0.04, 4)
+    b.run(0.04, 4)
+    np.testing.assert_array_equal(a.get_field(), b.get_field())
+    assert a.time() == b.time()
+    a.close()
+    b.close()
+
+
+def test_replicas_in_one_context(L, orc):
+    log("replicas")
+    sizes = [(8, 1), (16, 2), (12, 3)]
+    ctx = L.Context([L.model_params(N, seed=s, **dict(PAR, H=0.01 * s)) for N, s in sizes])
+    lats = []
+    for r, (N, s) in enumerate(sizes):
+        lat = orc.Lattice(N, seed=s, rng_mode=orc.RNG_PER_SITE, mass0=PAR["mass0"],
+                          lam=PAR["lam"], H=0.01 * s, chi=PAR["chi"])
+        u = np.random.default_rng(r).normal(size=(N, N, N, 10))
+        lat.u[...] = u
+        ctx.set_field(u, replica=r)
+        lats.append(lat)
+    ctx.step(0.04)
+    for r, lat in enumerate(lats):
+        lat.step(0.04)
+        assert rel_err(ctx.get_field(r), lat.u) <= 1e-12, r
+    ctx.close()
+
+
+def test_energy_tracking(L, orc):
+    log("energy")
+    N = 16
+    par = dict(PAR)
+    ctx = L.Context(L.model_params(N, seed=9, **par), track_energy=True)
+    lat = orc.Lattice(N, seed=9, rng_mode=orc.RNG_PER_SITE, mass0=par["mass0"],
+                      lam=par["lam"], H=par["H"], chi=par["chi"])
+    u = np.random.default_rng(1).normal(size=(N, N, N, 10)) * 0.5
+    lat.u[...] = u
+    ctx.set_field(u)
+    ctx.ideal_step(0.01)
+    lat.ideal_step(0.01)
+    e = ctx.energy()
+    np.testing.assert_allclose(e, list(lat.c.last_energy), rtol=1e-12)
+    ctx.close()
+
+
+def test_measure_walls(L, orc):
+    log("measure")
+    N = 16
+    ctx, lat = make_pair(L, orc, N, seed=2)
+    wall, wallphase, oavg = ctx.measure()
+    w, wp, oa = orc.measure_walls(lat.u)
+    np.testing.assert_allclose(wall, w, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(wallphase, wp, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(oavg, oa, rtol=1e-12, atol=1e-14)
+    ctx.close()
+
+
+def test_64_cubed_step(L, orc):
+    log("64^3")
+    N = 64
+    ctx, lat = make_pair(L, orc, N, seed=42, scale=0.5)
+    ctx.step(0.04)
+    lat.step(0.04)
+    assert rel_err(ctx.get_field(), lat.u) <= 1e-12
+    ctx.close()
