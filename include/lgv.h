@@ -153,6 +153,33 @@ int lgv_synchronize(lgv_ctx *ctx);
  * split into even and odd half-lattices) and its element size in bytes. */
 int lgv_field_device_ptr(lgv_ctx *ctx, int replica, void **ptr, int *elem_bytes);
 
+/* ------------------------------------------------------------------------
+ * Model A Euler--Maruyama stepper with a hand-written adjoint.  No
+ * counterpart in the reference (its Model A is the Metropolis
+ * EulerLangevinHB, Stepper.cxx:287-389; the forward-Euler recursion is the
+ * one of ModelG/freeCase.ipynb cells 5, 7).  Fields are caller-owned FP64
+ * device arrays phi[4][N][N][N] (a, z, y, x).  Noise is replayed from the
+ * per-site counter streams, so the backward pass sees the forward noise.
+ * The loss is mean_w(phibar) + N^3 var_w(phibar) over the last `window`
+ * steps, phibar_n the volume mean of phi_0 after step n. */
+typedef struct {
+  int N;
+  double m2, lambda, H, gamma, dt;
+  int seed;       /* seed_seq{seed} word -> per-site stream keys */
+  int noise_kind; /* 0: variance1 uniform (NoiseGenerator.h:105), 1: Box-Muller */
+} lgv_em_params;
+typedef struct lgv_em lgv_em;
+const char *lgv_em_last_error(void);
+int lgv_em_create(const lgv_em_params *p, int nsteps, int ckpt_every, int device,
+                  lgv_em **out);
+/* Launch on an external stream (e.g. torch's current stream); NULL = own. */
+int lgv_em_set_stream(lgv_em *h, void *hip_stream);
+int lgv_em_forward(lgv_em *h, const double *phi0, int window, double *phi_final,
+                   double *phibar_host, double *loss);
+/* d loss / d phi0 (device, same layout) and d loss / d (m2, H, lambda). */
+int lgv_em_backward(lgv_em *h, double *grad_phi0, double grad_params[3]);
+int lgv_em_destroy(lgv_em *h);
+
 #ifdef __cplusplus
 }
 #endif

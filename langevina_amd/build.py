@@ -15,7 +15,7 @@ LIB = os.path.join(PKG, "liblgv_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LGV_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["lgv_kernels.hip", "lgv_api.hip"]
+SOURCES = ["lgv_kernels.hip", "lgv_api.hip", "lgv_em.hip"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -86,8 +86,7 @@ struct lgv_ctx {
   ReplicaParams *dP = nullptr;
   ReplicaState *dS = nullptr;
   std::vector<void *> fields;
-  std::vector<double *> scratch;
-  Plan plan_half, plan_full;
+  Plan plan_half, plan_full, plan_ideal;
   MonSlot *mon_phi = nullptr, *mon_charge = nullptr, *mon_ideal = nullptr;
   EnergySlot *eslots = nullptr;
   double coef_hb = NAN, coef_charge = NAN, coef_ideal = NAN;
@@ -195,18 +194,17 @@ void enqueue_ideal(lgv_ctx *c, int off) {
   if (c->timing && c->ev_used + 16 > 4096) drain_timing(c);
   if (c->track_energy) {
     Timed tm(c, LGV_CLASS_ENERGY);
-    launch_energy(c->fp32, c->plan_full, c->dP, c->eslots, c->stream);
+    launch_energy(c->fp32, c->plan_full, c->dP, c->dS, c->eslots, c->stream);
     launch_energy_finish(c->nrep, c->plan_full, c->dP, c->dS, c->eslots, 0, off,
                          c->mon_ideal, c->stream);
   }
   {
     Timed tm(c, LGV_CLASS_IDEAL);
-    launch_ideal_rot1(c->fp32, c->plan_full, c->dP, c->stream);
-    launch_ideal_kick_rot2(c->fp32, c->plan_full, c->dP, c->stream);
+    launch_ideal(c->fp32, c->plan_ideal, c->dP, c->dS, c->nrep, c->stream);
   }
   if (c->track_energy) {
     Timed tm(c, LGV_CLASS_ENERGY);
-    launch_energy(c->fp32, c->plan_full, c->dP, c->eslots, c->stream);
+    launch_energy(c->fp32, c->plan_full, c->dP, c->dS, c->eslots, c->stream);
     launch_energy_finish(c->nrep, c->plan_full, c->dP, c->dS, c->eslots, 1, off,
                          c->mon_ideal, c->stream);
   }
@@ -297,23 +295,27 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
   c->hp.resize(c->nrep);
   c->noise.resize(c->nrep);
   const size_t esz = c->fp32 ? sizeof(float) : sizeof(double);
-  std::vector<int64_t> whalf, wfull;
+  std::vector<int64_t> whalf, wfull, wideal;
   std::vector<ReplicaState> hs(c->nrep);
   for (int r = 0; r < c->nrep; r++) {
     const lgv_model_params &m = c->mp[r];
     const int N = m.NX;
     const int64_t half = (int64_t)N * N * N / 2;
-    void *field = nullptr;
-    double *scr = nullptr;
-    HIPCHECK(hipMalloc(&field, 20 * half * esz));
-    HIPCHECK(hipMemset(field, 0, 20 * half * esz));
-    HIPCHECK(hipMalloc(&scr, 8 * half * sizeof(double)));
-    c->fields.push_back(field);
-    c->scratch.push_back(scr);
     ReplicaParams &p = c->hp[r];
     memset(&p, 0, sizeof p);
-    p.field = field;
-    p.scratch = scr;
+    for (int b = 0; b < 2; b++) {
+      void *field = nullptr;
+      HIPCHECK(hipMalloc(&field, 20 * half * esz));
+      HIPCHECK(hipMemset(field, 0, 20 * half * esz));
+      c->fields.push_back(field);
+      p.fields[b] = field;
+    }
+    int TXH, TY, TZ;
+    ideal_tile(N, TXH, TY, TZ);
+    if ((TY + 2) * (2 * TXH + 2) > kIdealLds)
+      return fail(LGV_ERR_ARG, "replica %d: NX=%d gives an ideal tile beyond LDS", r, N);
+    wideal.push_back((int64_t)((N / 2 + TXH - 1) / TXH) * ((N + TY - 1) / TY) *
+                     ((N + TZ - 1) / TZ) * kBlock);
     p.N = N;
     p.NH = N / 2;
     p.half = half;
@@ -343,6 +345,7 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
   int rc;
   if ((rc = build_plan(c->plan_half, whalf)) != LGV_OK) return rc;
   if ((rc = build_plan(c->plan_full, wfull)) != LGV_OK) return rc;
+  if ((rc = build_plan(c->plan_ideal, wideal)) != LGV_OK) return rc;
   HIPCHECK(hipMalloc(&c->mon_phi, c->plan_half.nblocks * sizeof(MonSlot)));
   HIPCHECK(hipMemset(c->mon_phi, 0, c->plan_half.nblocks * sizeof(MonSlot)));
   HIPCHECK(hipMalloc(&c->mon_charge, c->plan_half.nblocks * sizeof(MonSlot)));
@@ -368,13 +371,12 @@ int lgv_destroy(lgv_ctx *c) {
   if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
   for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
   for (void *f : c->fields) hipFree(f);
-  for (double *s : c->scratch) hipFree(s);
   hipFree(c->dP);
   hipFree(c->dS);
-  hipFree(c->plan_half.d_blk_rep);
-  hipFree(c->plan_half.d_blk_first);
-  hipFree(c->plan_full.d_blk_rep);
-  hipFree(c->plan_full.d_blk_first);
+  for (Plan *pl : {&c->plan_half, &c->plan_full, &c->plan_ideal}) {
+    hipFree(pl->d_blk_rep);
+    hipFree(pl->d_blk_first);
+  }
   hipFree(c->mon_phi);
   hipFree(c->mon_charge);
   hipFree(c->mon_ideal);
@@ -398,7 +400,7 @@ int lgv_set_field(lgv_ctx *c, int r, const double *aos) {
   const int64_t V = (int64_t)N * N * N;
   HIPCHECK(hipStreamSynchronize(c->stream));
   HIPCHECK(hipMemcpy(c->d_aos, aos, V * 10 * sizeof(double), hipMemcpyHostToDevice));
-  launch_aos_to_rb(c->fp32, c->dP, r, N, c->d_aos, c->stream);
+  launch_aos_to_rb(c->fp32, c->dP, c->dS, r, N, c->d_aos, c->stream);
   if ((rc = check_launch())) return rc;
   HIPCHECK(hipStreamSynchronize(c->stream));
   return LGV_OK;
@@ -411,7 +413,7 @@ int lgv_get_field(lgv_ctx *c, int r, double *aos) {
   HIPCHECK(hipSetDevice(c->device));
   const int N = c->hp[r].N;
   const int64_t V = (int64_t)N * N * N;
-  launch_rb_to_aos(c->fp32, c->dP, r, N, c->d_aos, c->stream);
+  launch_rb_to_aos(c->fp32, c->dP, c->dS, r, N, c->d_aos, c->stream);
   if ((rc = check_launch())) return rc;
   HIPCHECK(hipStreamSynchronize(c->stream));
   HIPCHECK(hipMemcpy(aos, c->d_aos, V * 10 * sizeof(double), hipMemcpyDeviceToHost));
@@ -629,7 +631,7 @@ int lgv_measure(lgv_ctx *c, int r, double *wall, double *wallphase, double *oavg
   if (!wall || !wallphase || !oavg) return fail(LGV_ERR_ARG, "null output");
   HIPCHECK(hipSetDevice(c->device));
   const int N = c->hp[r].N;
-  launch_measure(c->fp32, c->dP, r, N, c->d_partXZ, c->d_partXY, c->d_wall, c->d_wallphase,
+  launch_measure(c->fp32, c->dP, c->dS, r, N, c->d_partXZ, c->d_partXY, c->d_wall, c->d_wallphase,
                  c->stream);
   if ((rc = check_launch())) return rc;
   HIPCHECK(hipStreamSynchronize(c->stream));

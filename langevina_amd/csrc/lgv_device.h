@@ -189,6 +189,44 @@ LGV_HD void rotate_site(double phi[4], const double A[3], const double V[3],
   phi[0] = out[0]; phi[1] = out[1]; phi[2] = out[2]; phi[3] = out[3];
 }
 
+// The same rotation for the fused ideal kernel.  With k = tau/chi the two
+// generators are tmp1 = -0.5(mu_V - mu_A) = h (V - A) and
+// tmp2 = 0.5(mu_V + mu_A) = -h (V + A), h = k/2.  expmul needs cos|v| and
+// sin|v|/|v|; for |v| < 1/2 both are evaluated as Taylor series in
+// y = |v|^2 (truncation below 1e-18), otherwise with sqrt/sincos.  Agrees
+// with rotate_site to a few ulps (the reference's own libm calls are not
+// bit-reproducible on the device anyway).
+LGV_HD void cos_sinc(double y, double &c, double &s) {
+  if (y < 0.25) {
+    c = 1.0 + y * (-1.0 / 2 + y * (1.0 / 24 + y * (-1.0 / 720 + y * (1.0 / 40320 +
+        y * (-1.0 / 3628800 + y * (1.0 / 479001600 + y * (-1.0 / 87178291200.0)))))));
+    s = 1.0 + y * (-1.0 / 6 + y * (1.0 / 120 + y * (-1.0 / 5040 + y * (1.0 / 362880 +
+        y * (-1.0 / 39916800 + y * (1.0 / 6227020800.0 + y * (-1.0 / 1307674368000.0)))))));
+  } else {
+    const double a = sqrt(y);
+    double sn, cs;
+    sincos(a, &sn, &cs);
+    c = cs;
+    s = sn / a;
+  }
+}
+
+LGV_HD void rotate_site_fast(double phi[4], const double A[3], const double V[3], double h) {
+  double r[3], l[3];
+  for (int i = 0; i < 3; i++) {
+    r[i] = h * (V[i] - A[i]);
+    l[i] = -h * (V[i] + A[i]);
+  }
+  double cR, sR, cL, sL;
+  cos_sinc(r[0] * r[0] + r[1] * r[1] + r[2] * r[2], cR, sR);
+  cos_sinc(l[0] * l[0] + l[1] * l[1] + l[2] * l[2], cL, sL);
+  const double R[4] = {cR, -sR * r[0], -sR * r[1], -sR * r[2]};
+  const double L[4] = {cL, -sL * l[0], -sL * l[1], -sL * l[2]};
+  double t[4];
+  quatmul(phi, R, t);
+  quatmul(L, t, phi);
+}
+
 // ---------------------------------------------------------------- sums
 // Compensated (TwoSum / Neumaier) accumulation for FP64 reductions.
 struct KSum {

@@ -1,0 +1,423 @@
+// lgv_em.hip -- Model A Euler--Maruyama stepper and its adjoint (new
+// functionality: the reference has no EM stepper; its Metropolis "Model A"
+// is EulerLangevinHB, ModelG/Stepper.cxx:287-389, and the forward-Euler
+// recursion is the one of ModelG/freeCase.ipynb cells 5, 7).
+//
+//   phi_{n+1} = phi_n - G dt (-Lap phi_n + m2 phi_n + lambda |phi_n|^2 phi_n - H e0)
+//               + sqrt(2 G dt) xi_n
+//
+// with the reference's lattice Laplacian (six neighbours minus 6 phi) and
+// xi_n from per-site counter streams (replayed exactly in the backward
+// pass).  Fields are caller-owned device arrays phi[a][z][y][x] (FP64).
+// The loss is  mean_w(phibar) + V var_w(phibar)  over the last `window`
+// steps, phibar_n = volume mean of phi_0 after step n.  The adjoint uses
+// the symmetric one-step Jacobian J = I - G dt (-Lap + m2 + lambda (|phi|^2
+// + 2 phi phi^T)), checkpoints every K steps and recomputation inside a
+// segment.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "lgv.h"
+#include "lgv_device.h"
+
+using namespace lgv;
+
+namespace {
+
+constexpr int kEmBlock = 256;
+constexpr uint64_t kEmTag = 0x4000000000000000ULL;
+
+struct GradSlot {
+  double m2_s, m2_c, h_s, h_c, l_s, l_c;
+};
+
+__device__ __forceinline__ int wp(int a, int N) { return a == N ? 0 : a; }
+__device__ __forceinline__ int wm(int a, int N) { return a < 0 ? N - 1 : a; }
+
+__device__ __forceinline__ void site_xyz(int64_t s, int N, int &x, int &y, int &z) {
+  x = (int)(s % N);
+  const int64_t yz = s / N;
+  y = (int)(yz % N);
+  z = (int)(yz / N);
+}
+
+__device__ __forceinline__ void em_noise(uint64_t skey, int64_t s, int kind, double xi[4]) {
+  Xoro g = xoro_seeded(site_key(skey, (uint64_t)s));
+  if (kind == 0) {
+    for (int a = 0; a < 4; a++) xi[a] = variance1(g);
+  } else {
+    for (int a = 0; a < 4; a += 2) {
+      const double u1 = canonical(xoro_next(g));
+      const double u2 = canonical(xoro_next(g));
+      const double r = sqrt(-2.0 * log(1.0 - u1));
+      const double th = 6.283185307179586 * u2;
+      xi[a] = r * cos(th);
+      xi[a + 1] = r * sin(th);
+    }
+  }
+}
+
+__device__ void block_ksum_store(KSum k, double *out2) {
+  for (int o = 32; o > 0; o >>= 1) {
+    KSum t;
+    t.s = __shfl_xor(k.s, o, 64);
+    t.c = __shfl_xor(k.c, o, 64);
+    ksum_merge(k, t);
+  }
+  __shared__ KSum sh[kEmBlock / 64];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[w] = k;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    KSum a = sh[0];
+    for (int i = 1; i < kEmBlock / 64; i++) ksum_merge(a, sh[i]);
+    out2[0] = a.s;
+    out2[1] = a.c;
+  }
+}
+
+// One forward step in -> out; per-block compensated partial of sum out[0].
+__global__ __launch_bounds__(kEmBlock) void k_em_step(const double *__restrict__ in,
+                                                      double *__restrict__ out, int N,
+                                                      double m2, double lam, double H,
+                                                      double gdt, double amp, uint64_t skey,
+                                                      int kind, double *__restrict__ part) {
+  const int64_t V = (int64_t)N * N * N;
+  const int64_t s = (int64_t)blockIdx.x * kEmBlock + threadIdx.x;
+  KSum acc{0., 0.};
+  if (s < V) {
+    int x, y, z;
+    site_xyz(s, N, x, y, z);
+    const int64_t ixp = s - x + wp(x + 1, N), ixm = s - x + wm(x - 1, N);
+    const int64_t iyp = s + (int64_t)(wp(y + 1, N) - y) * N, iym = s + (int64_t)(wm(y - 1, N) - y) * N;
+    const int64_t izp = s + (int64_t)(wp(z + 1, N) - z) * N * N;
+    const int64_t izm = s + (int64_t)(wm(z - 1, N) - z) * N * N;
+    double c[4];
+    for (int a = 0; a < 4; a++) c[a] = in[a * V + s];
+    const double s2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2] + c[3] * c[3];
+    double xi[4];
+    em_noise(skey, s, kind, xi);
+    for (int a = 0; a < 4; a++) {
+      const double *f = in + a * V;
+      const double nb = (f[ixp] + f[ixm]) + (f[iyp] + f[iym]) + (f[izp] + f[izm]);
+      const double lap = nb - 6.0 * c[a];
+      double drift = (-lap + m2 * c[a]) + (lam * s2) * c[a];
+      if (a == 0) drift = drift - H;
+      const double o = (c[a] - gdt * drift) + amp * xi[a];
+      out[a * V + s] = o;
+      if (a == 0) acc.s = o;
+    }
+  }
+  block_ksum_store(acc, part + 2 * blockIdx.x);
+}
+
+// phibar[n] = (sum of the block partials) / V
+__global__ void k_em_phibar(const double *part, int nblocks, int64_t V, double *phibar, int n) {
+  KSum acc{0., 0.};
+  for (int b = threadIdx.x; b < nblocks; b += 64) ksum_merge(acc, KSum{part[2 * b], part[2 * b + 1]});
+  for (int o = 32; o > 0; o >>= 1) {
+    KSum t;
+    t.s = __shfl_xor(acc.s, o, 64);
+    t.c = __shfl_xor(acc.c, o, 64);
+    ksum_merge(acc, t);
+  }
+  if (threadIdx.x == 0) phibar[n] = ksum_value(acc) / (double)V;
+}
+
+// One adjoint step: lam_out = J(phi)^T (lam_in + dl e0); accumulates the
+// parameter gradients into the block's slot.
+__global__ __launch_bounds__(kEmBlock) void k_em_adjoint(const double *__restrict__ phi,
+                                                         const double *__restrict__ lin,
+                                                         double *__restrict__ lout, int N,
+                                                         double m2, double lam, double gdt,
+                                                         double dl,
+                                                         GradSlot *__restrict__ slots) {
+  const int64_t V = (int64_t)N * N * N;
+  const int64_t s = (int64_t)blockIdx.x * kEmBlock + threadIdx.x;
+  double gm2 = 0., gl = 0., gh = 0.;
+  if (s < V) {
+    int x, y, z;
+    site_xyz(s, N, x, y, z);
+    const int64_t ixp = s - x + wp(x + 1, N), ixm = s - x + wm(x - 1, N);
+    const int64_t iyp = s + (int64_t)(wp(y + 1, N) - y) * N, iym = s + (int64_t)(wm(y - 1, N) - y) * N;
+    const int64_t izp = s + (int64_t)(wp(z + 1, N) - z) * N * N;
+    const int64_t izm = s + (int64_t)(wm(z - 1, N) - z) * N * N;
+    double c[4], l[4];
+    for (int a = 0; a < 4; a++) {
+      c[a] = phi[a * V + s];
+      l[a] = lin[a * V + s] + (a == 0 ? dl : 0.0);
+    }
+    const double s2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2] + c[3] * c[3];
+    const double pl = c[0] * l[0] + c[1] * l[1] + c[2] * l[2] + c[3] * l[3];
+    for (int a = 0; a < 4; a++) {
+      const double *f = lin + a * V;
+      const double e = a == 0 ? dl : 0.0;
+      const double nb = ((f[ixp] + e) + (f[ixm] + e)) + ((f[iyp] + e) + (f[iym] + e)) +
+                        ((f[izp] + e) + (f[izm] + e));
+      const double lapl = nb - 6.0 * l[a];
+      const double jt = (-lapl + m2 * l[a]) + lam * (s2 * l[a] + 2.0 * c[a] * pl);
+      lout[a * V + s] = l[a] - gdt * jt;
+    }
+    gm2 = -gdt * pl;
+    gl = -gdt * s2 * pl;
+    gh = gdt * l[0];
+  }
+  KSum a{gm2, 0.}, b{gh, 0.}, cc{gl, 0.};
+  for (int o = 32; o > 0; o >>= 1) {
+    KSum t;
+    t.s = __shfl_xor(a.s, o, 64); t.c = __shfl_xor(a.c, o, 64); ksum_merge(a, t);
+    t.s = __shfl_xor(b.s, o, 64); t.c = __shfl_xor(b.c, o, 64); ksum_merge(b, t);
+    t.s = __shfl_xor(cc.s, o, 64); t.c = __shfl_xor(cc.c, o, 64); ksum_merge(cc, t);
+  }
+  __shared__ KSum sh[3][kEmBlock / 64];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { sh[0][w] = a; sh[1][w] = b; sh[2][w] = cc; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    KSum A = sh[0][0], B = sh[1][0], Cc = sh[2][0];
+    for (int i = 1; i < kEmBlock / 64; i++) {
+      ksum_merge(A, sh[0][i]); ksum_merge(B, sh[1][i]); ksum_merge(Cc, sh[2][i]);
+    }
+    GradSlot &g = slots[blockIdx.x];
+    KSum m{g.m2_s, g.m2_c}, h{g.h_s, g.h_c}, l{g.l_s, g.l_c};
+    ksum_merge(m, A); ksum_merge(h, B); ksum_merge(l, Cc);
+    g = GradSlot{m.s, m.c, h.s, h.c, l.s, l.c};
+  }
+}
+
+thread_local std::string g_em_err;
+int em_fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_em_err = buf;
+  return code;
+}
+#define EMCHECK(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return em_fail(LGV_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                     __FILE__, __LINE__);                                          \
+  } while (0)
+
+}  // namespace
+
+struct lgv_em {
+  lgv_em_params p;
+  int nsteps = 0, K = 0, nseg = 0, window = 0, device = 0;
+  int64_t V = 0;
+  int nblocks = 0;
+  uint64_t base = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  double *ckpt = nullptr;   // nseg states
+  double *seg = nullptr;    // K states of the segment being reversed
+  double *work[2] = {nullptr, nullptr};
+  double *part = nullptr;   // per-block partial sums
+  double *phibar = nullptr; // nsteps
+  GradSlot *gslots = nullptr;
+  std::vector<double> phibar_host;
+  bool have_forward = false;
+};
+
+namespace {
+
+// Steps n0..n1-1 starting from `start`, recording phibar of every step.
+// Forward mode (states == nullptr): ping-pongs through the work buffers and
+// leaves phi_{n1} in `last`.  Recompute mode: states[i] = phi_{n0+i} for
+// i = 0..n1-n0 (start is copied to states[0]); the work buffers, which hold
+// the adjoint state during the backward pass, are not touched.
+int em_run_segment(lgv_em *h, const double *start, int n0, int n1, double *states,
+                   double *last) {
+  const double gdt = h->p.gamma * h->p.dt;
+  const double amp = std::sqrt(2.0 * gdt);
+  const size_t st = 4 * h->V * sizeof(double);
+  const double *in = start;
+  if (states) {
+    EMCHECK(hipMemcpyAsync(states, start, st, hipMemcpyDeviceToDevice, h->stream));
+    in = states;
+  }
+  for (int n = n0; n < n1; n++) {
+    double *out = states ? states + (int64_t)(n - n0 + 1) * 4 * h->V
+                         : ((n + 1 == n1) ? last : h->work[(n - n0) & 1]);
+    const uint64_t skey = stream_key(h->base, kEmTag | (uint64_t)n);
+    k_em_step<<<h->nblocks, kEmBlock, 0, h->stream>>>(in, out, h->p.N, h->p.m2, h->p.lambda,
+                                                      h->p.H, gdt, amp, skey, h->p.noise_kind,
+                                                      h->part);
+    k_em_phibar<<<1, 64, 0, h->stream>>>(h->part, h->nblocks, h->V, h->phibar, n);
+    in = out;
+  }
+  if (!states && n1 == n0 && last != start)
+    EMCHECK(hipMemcpyAsync(last, start, st, hipMemcpyDeviceToDevice, h->stream));
+  EMCHECK(hipGetLastError());
+  return LGV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lgv_em_last_error(void) { return g_em_err.c_str(); }
+
+int lgv_em_create(const lgv_em_params *p, int nsteps, int ckpt_every, int device,
+                  lgv_em **out) {
+  if (!p || !out) return em_fail(LGV_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (p->N < 4 || p->N > 1024) return em_fail(LGV_ERR_ARG, "N=%d outside [4,1024]", p->N);
+  if (nsteps <= 0 || ckpt_every <= 0) return em_fail(LGV_ERR_ARG, "nsteps and ckpt_every must be positive");
+  if (p->noise_kind != 0 && p->noise_kind != 1) return em_fail(LGV_ERR_ARG, "noise_kind must be 0 or 1");
+  lgv_em *h = new lgv_em;
+  h->p = *p;
+  h->nsteps = nsteps;
+  h->K = ckpt_every;
+  h->nseg = (nsteps + ckpt_every - 1) / ckpt_every;
+  h->device = device;
+  h->V = (int64_t)p->N * p->N * p->N;
+  h->nblocks = (int)((h->V + kEmBlock - 1) / kEmBlock);
+  std::seed_seq seq = {p->seed};
+  uint32_t s32 = 0;
+  seq.generate(&s32, &s32 + 1);
+  h->base = base_key(s32);
+  const size_t st = 4 * h->V * sizeof(double);
+  auto cleanup_fail = [&](hipError_t e) {
+    lgv_em_destroy(h);
+    return em_fail(LGV_ERR_HIP, "lgv_em_create: %s", hipGetErrorString(e));
+  };
+  hipError_t e;
+  if ((e = hipSetDevice(device)) != hipSuccess) return cleanup_fail(e);
+  if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess) return cleanup_fail(e);
+  h->own_stream = true;
+  if ((e = hipMalloc(&h->ckpt, st * h->nseg)) != hipSuccess) return cleanup_fail(e);
+  if ((e = hipMalloc(&h->seg, st * h->K)) != hipSuccess) return cleanup_fail(e);
+  for (int i = 0; i < 2; i++)
+    if ((e = hipMalloc(&h->work[i], st)) != hipSuccess) return cleanup_fail(e);
+  if ((e = hipMalloc(&h->part, 2 * h->nblocks * sizeof(double))) != hipSuccess) return cleanup_fail(e);
+  if ((e = hipMalloc(&h->phibar, nsteps * sizeof(double))) != hipSuccess) return cleanup_fail(e);
+  if ((e = hipMalloc(&h->gslots, h->nblocks * sizeof(GradSlot))) != hipSuccess) return cleanup_fail(e);
+  *out = h;
+  return LGV_OK;
+}
+
+int lgv_em_set_stream(lgv_em *h, void *stream) {
+  if (!h) return em_fail(LGV_ERR_ARG, "null handle");
+  if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
+  h->own_stream = stream == nullptr;
+  if (stream) {
+    h->stream = (hipStream_t)stream;
+  } else {
+    EMCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  }
+  return LGV_OK;
+}
+
+int lgv_em_forward(lgv_em *h, const double *phi0, int window, double *phi_final,
+                   double *phibar_host, double *loss) {
+  if (!h || !phi0 || !loss) return em_fail(LGV_ERR_ARG, "null argument");
+  if (window <= 0 || window > h->nsteps) return em_fail(LGV_ERR_ARG, "window outside [1,nsteps]");
+  EMCHECK(hipSetDevice(h->device));
+  h->window = window;
+  const size_t st = 4 * h->V * sizeof(double);
+  EMCHECK(hipMemcpyAsync(h->ckpt, phi0, st, hipMemcpyDeviceToDevice, h->stream));
+  int rc;
+  for (int j = 0; j < h->nseg; j++) {
+    const int n0 = j * h->K, n1 = std::min(h->nsteps, n0 + h->K);
+    double *last = (j + 1 < h->nseg) ? h->ckpt + (int64_t)(j + 1) * 4 * h->V
+                                     : (phi_final ? phi_final : h->seg);
+    if ((rc = em_run_segment(h, h->ckpt + (int64_t)j * 4 * h->V, n0, n1, nullptr, last)))
+      return rc;
+  }
+  h->phibar_host.resize(h->nsteps);
+  EMCHECK(hipMemcpyAsync(h->phibar_host.data(), h->phibar, h->nsteps * sizeof(double),
+                         hipMemcpyDeviceToHost, h->stream));
+  EMCHECK(hipStreamSynchronize(h->stream));
+  long double m = 0, m2 = 0;
+  for (int n = h->nsteps - window; n < h->nsteps; n++) {
+    m += h->phibar_host[n];
+    m2 += (long double)h->phibar_host[n] * h->phibar_host[n];
+  }
+  m /= window;
+  m2 /= window;
+  *loss = (double)(m + (long double)h->V * (m2 - m * m));
+  if (phibar_host) memcpy(phibar_host, h->phibar_host.data(), h->nsteps * sizeof(double));
+  h->have_forward = true;
+  return LGV_OK;
+}
+
+int lgv_em_backward(lgv_em *h, double *grad_phi0, double grad_params[3]) {
+  if (!h || !grad_phi0 || !grad_params) return em_fail(LGV_ERR_ARG, "null argument");
+  if (!h->have_forward) return em_fail(LGV_ERR_STATE, "lgv_em_backward before lgv_em_forward");
+  EMCHECK(hipSetDevice(h->device));
+  const size_t st = 4 * h->V * sizeof(double);
+  const int W = h->window;
+  long double mean = 0;
+  for (int n = h->nsteps - W; n < h->nsteps; n++) mean += h->phibar_host[n];
+  mean /= W;
+  const double gdt = h->p.gamma * h->p.dt;
+  double *lam = h->work[0], *nxt = h->work[1];
+  EMCHECK(hipMemsetAsync(lam, 0, st, h->stream));
+  EMCHECK(hipMemsetAsync(h->gslots, 0, h->nblocks * sizeof(GradSlot), h->stream));
+  int rc;
+  for (int j = h->nseg - 1; j >= 0; j--) {
+    const int n0 = j * h->K, n1 = std::min(h->nsteps, n0 + h->K);
+    // recompute phi_{n0} .. phi_{n1-1} into seg (the forward noise is
+    // replayed from the same counters)
+    if ((rc = em_run_segment(h, h->ckpt + (int64_t)j * 4 * h->V, n0, n1 - 1, h->seg,
+                             nullptr)))
+      return rc;
+    for (int n = n1 - 1; n >= n0; n--) {
+      double dl = 0.;
+      if (n >= h->nsteps - W)
+        dl = (1.0 / W + (double)h->V * 2.0 * (h->phibar_host[n] - (double)mean) / W) /
+             (double)h->V;
+      k_em_adjoint<<<h->nblocks, kEmBlock, 0, h->stream>>>(
+          h->seg + (int64_t)(n - n0) * 4 * h->V, lam, nxt, h->p.N, h->p.m2, h->p.lambda, gdt,
+          dl, h->gslots);
+      std::swap(lam, nxt);
+    }
+  }
+  EMCHECK(hipGetLastError());
+  EMCHECK(hipMemcpyAsync(grad_phi0, lam, st, hipMemcpyDeviceToDevice, h->stream));
+  std::vector<GradSlot> gs(h->nblocks);
+  EMCHECK(hipMemcpyAsync(gs.data(), h->gslots, h->nblocks * sizeof(GradSlot),
+                         hipMemcpyDeviceToHost, h->stream));
+  EMCHECK(hipStreamSynchronize(h->stream));
+  KSum m{0, 0}, hh{0, 0}, l{0, 0};
+  for (const GradSlot &g : gs) {
+    ksum_merge(m, KSum{g.m2_s, g.m2_c});
+    ksum_merge(hh, KSum{g.h_s, g.h_c});
+    ksum_merge(l, KSum{g.l_s, g.l_c});
+  }
+  grad_params[0] = ksum_value(m);
+  grad_params[1] = ksum_value(hh);
+  grad_params[2] = ksum_value(l);
+  // work[0/1] were used as lam buffers; restore the pointer roles
+  h->work[0] = lam;
+  h->work[1] = nxt;
+  return LGV_OK;
+}
+
+int lgv_em_destroy(lgv_em *h) {
+  if (!h) return LGV_OK;
+  hipSetDevice(h->device);
+  if (h->stream) hipStreamSynchronize(h->stream);
+  hipFree(h->ckpt);
+  hipFree(h->seg);
+  hipFree(h->work[0]);
+  hipFree(h->work[1]);
+  hipFree(h->part);
+  hipFree(h->phibar);
+  hipFree(h->gslots);
+  if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return LGV_OK;
+}
+
+}  // extern "C"

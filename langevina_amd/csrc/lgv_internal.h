@@ -11,8 +11,9 @@ namespace lgv {
 // coefficients (rdtg, rms, dt_ideal, tau) are computed on the host with the
 // reference's expressions for the dt in use.
 struct ReplicaParams {
-  void *field;        // 10 planes x 2 parities x half elements (T = double/float)
-  double *scratch;    // 4 x 2 x half doubles: rotated phi inside IdealPV2
+  void *fields[2];    // two buffers of 10 planes x 2 parities x half elements
+                      // (T = double/float); ReplicaState::cur selects the live
+                      // one, the ideal substep writes the other
   int N, NH;
   int64_t half;       // N^3 / 2
   double mass0, dmassdt, lambda, H, chi;
@@ -29,9 +30,20 @@ struct ReplicaState {
   double t;
   uint64_t sweep_ctr;
   int order[5];      // charge sub-sweep order: directions[3], parities[2]
-  int pad;
+  int cur;           // live field buffer (0/1)
   double E_old, E_new;
 };
+
+// Tile of the fused ideal kernel: TXH half-columns x TY rows in x-y,
+// marching over TZ planes in z.  (TY+2)(2 TXH+2) <= kIdealLds.
+constexpr int kIdealLds = 680;
+__host__ __device__ inline void ideal_tile(int N, int &TXH, int &TY, int &TZ) {
+  const int NH = N / 2;
+  TXH = NH < 32 ? NH : 32;
+  TY = 256 / TXH;
+  if (TY > N) TY = N;
+  TZ = N < 16 ? N : 16;
+}
 
 // Per-block monitor partials (o4_stepper_monitor counters).
 struct MonSlot {
@@ -60,22 +72,22 @@ void launch_shuffle(int nrep, ReplicaState *S, const ReplicaParams *P, int ctr_o
                     hipStream_t st);
 void launch_charge(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
                    int sub, int ctr_off, MonSlot *mon, hipStream_t st);
-void launch_ideal_rot1(bool fp32, const Plan &plan, const ReplicaParams *P, hipStream_t st);
-void launch_ideal_kick_rot2(bool fp32, const Plan &plan, const ReplicaParams *P,
-                            hipStream_t st);
-void launch_energy(bool fp32, const Plan &plan, const ReplicaParams *P, EnergySlot *slots,
-                   hipStream_t st);
+void launch_ideal(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
+                  int nrep, hipStream_t st);
+void launch_energy(bool fp32, const Plan &plan, const ReplicaParams *P,
+                   const ReplicaState *S, EnergySlot *slots, hipStream_t st);
 void launch_energy_finish(int nrep, const Plan &plan, const ReplicaParams *P,
                           ReplicaState *S, const EnergySlot *slots, int which,
                           int ctr_off, MonSlot *mon_ideal, hipStream_t st);
 void launch_advance(int nrep, ReplicaState *S, double dt, int advance_time,
                     uint64_t nctr, hipStream_t st);
 void launch_set_time(int nrep, int replica, ReplicaState *S, double t, hipStream_t st);
-void launch_aos_to_rb(bool fp32, const ReplicaParams *P, int replica, int N,
-                      const double *aos, hipStream_t st);
-void launch_rb_to_aos(bool fp32, const ReplicaParams *P, int replica, int N, double *aos,
-                      hipStream_t st);
-void launch_measure(bool fp32, const ReplicaParams *P, int replica, int N, double *partXZ,
-                    double *partXY, double *wall, double *wallphase, hipStream_t st);
+void launch_aos_to_rb(bool fp32, const ReplicaParams *P, const ReplicaState *S, int replica,
+                      int N, const double *aos, hipStream_t st);
+void launch_rb_to_aos(bool fp32, const ReplicaParams *P, const ReplicaState *S, int replica,
+                      int N, double *aos, hipStream_t st);
+void launch_measure(bool fp32, const ReplicaParams *P, const ReplicaState *S, int replica,
+                    int N, double *partXZ, double *partXY, double *wall, double *wallphase,
+                    hipStream_t st);
 
 }  // namespace lgv

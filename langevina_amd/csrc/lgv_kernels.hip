@@ -1,11 +1,13 @@
 // lgv_kernels.hip -- HIP kernels of the ModelG time step for gfx950.
 //
-// One thread per active site, 256-thread blocks (4 wave64s), blocks mapped
-// to replicas through a launch plan, so that one launch steps many
-// independent lattices.  Fields live in the red-black structure-of-arrays
-// layout of lgv_device.h.  All FP arithmetic is FP64 (FP32 storage is
-// widened on load and rounded on store) and follows the reference's
-// expression order; the library is built with -ffp-contract=off.
+// Blocks are mapped to replicas through a launch plan, so one launch steps
+// many independent lattices.  Fields live in the red-black
+// structure-of-arrays layout of lgv_device.h, double-buffered: the
+// Metropolis sweeps update the live buffer in place (a checkerboard never
+// reads what it writes), the fused ideal substep writes the other buffer.
+// All FP arithmetic is FP64 (FP32 storage is widened on load and rounded
+// on store).  The Metropolis sweeps follow the reference's expression order
+// bit for bit; the library is built with -ffp-contract=off.
 #include <float.h>
 
 #include "lgv_device.h"
@@ -13,9 +15,12 @@
 
 namespace lgv {
 
+// A select instead of p.fields[cur]: a runtime index into the register copy
+// of ReplicaParams would push the whole struct to scratch memory.
 template <typename T>
-__device__ __forceinline__ T *plane(const ReplicaParams &p, int c, int par) {
-  return reinterpret_cast<T *>(p.field) + ((int64_t)(c * 2 + par)) * p.half;
+__device__ __forceinline__ T *plane(const ReplicaParams &p, int cur, int c, int par) {
+  T *base = reinterpret_cast<T *>(cur ? p.fields[1] : p.fields[0]);
+  return base + ((int64_t)(c * 2 + par)) * p.half;
 }
 
 __device__ __forceinline__ int wrapp(int a, int N) { return a == N ? 0 : a; }
@@ -94,11 +99,11 @@ __global__ __launch_bounds__(kBlock) void k_hb_phi(const ReplicaParams *__restri
     double phi_o[4], phi_n[4];
 #pragma unroll
     for (int l = 0; l < 4; l++) {
-      const T *fq = plane<T>(p, l, q);
+      const T *fq = plane<T>(p, s.cur, l, q);
       const double heff = ((double)fq[nb[0]] + (double)fq[nb[1]]) +
                           ((double)fq[nb[2]] + (double)fq[nb[3]]) +
                           ((double)fq[nb[4]] + (double)fq[nb[5]]);
-      phi_o[l] = (double)plane<T>(p, l, ieo)[idx];
+      phi_o[l] = (double)plane<T>(p, s.cur, l, ieo)[idx];
       phi_n[l] = phi_o[l] + rdtg * variance1(g);
       s_o += phi_o[l] * phi_o[l];
       s_n += phi_n[l] * phi_n[l];
@@ -121,7 +126,7 @@ __global__ __launch_bounds__(kBlock) void k_hb_phi(const ReplicaParams *__restri
     }
     if (accept) {
 #pragma unroll
-      for (int l = 0; l < 4; l++) plane<T>(p, l, ieo)[idx] = (T)phi_n[l];
+      for (int l = 0; l < 4; l++) plane<T>(p, s.cur, l, ieo)[idx] = (T)phi_n[l];
     }
   }
   block_monitor_add(mon + blockIdx.x, down, upyes, upno, dSd, dSu);
@@ -156,8 +161,10 @@ __global__ __launch_bounds__(kBlock) void k_charge(const ReplicaParams *__restri
   double dSd = 0., dSu = 0.;
   if (idx < p.half) {
     const ReplicaState s = S[r];
-    const int dir = s.order[sub >> 1];
-    const int eoA = s.order[3 + (sub & 1)];
+    // indexed loads from global memory: a runtime index into the private
+    // copy would make the compiler stage the struct in LDS
+    const int dir = S[r].order[sub >> 1];
+    const int eoA = S[r].order[3 + (sub & 1)];
     const int N = p.N, NH = p.NH;
     const int xh = (int)(idx % NH);
     const int64_t rowi = idx / NH;
@@ -172,8 +179,8 @@ __global__ __launch_bounds__(kBlock) void k_charge(const ReplicaParams *__restri
                                   ((uint64_t)z * N + y) * N + x));
 #pragma unroll
     for (int L = 4; L < 10; L++) {
-      T *pa = plane<T>(p, L, eoA);
-      T *pb = plane<T>(p, L, eoA ^ 1);
+      T *pa = plane<T>(p, s.cur, L, eoA);
+      T *pb = plane<T>(p, s.cur, L, eoA ^ 1);
       const double nA = (double)pa[idx], nB = (double)pb[idxB];
       double q = rms * variance1(g);
       const double dS = ((nA - q) * (nA - q) + (nB + q) * (nB + q) - nA * nA - nB * nB) /
@@ -199,87 +206,171 @@ __global__ __launch_bounds__(kBlock) void k_charge(const ReplicaParams *__restri
 }
 
 // ------------------------------------------------------------------------
-// IdealPV2::step_no_reject (Stepper.cxx:68-176), split in two passes:
-// rot1 rotates every site by dt/2 into an FP64 scratch copy of phi,
-// kick_rot2 kicks the charges with the rotated neighbours and rotates again.
+// IdealPV2::step_no_reject (Stepper.cxx:68-176) fused into one pass:
+// rotate by dt/2, kick the charges with the rotated neighbours, rotate by
+// dt/2 with the kicked charges.  A block owns a TXH x TY x-y tile (both
+// parities of each half-column) and marches over TZ planes; the rotated phi
+// of the current plane and a one-site x-y halo (recomputed from the raw
+// fields) sit in LDS, the rotated phi of the planes below and above in
+// registers.  Reads the live buffer, writes the other one.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_ideal_rot1(const ReplicaParams *__restrict__ P,
-                                                       const int *__restrict__ blk_rep,
-                                                       const int *__restrict__ blk_first) {
-  const int r = blk_rep[blockIdx.x];
-  const ReplicaParams p = P[r];
-  int64_t idx = (int64_t)(blockIdx.x - blk_first[r]) * kBlock + threadIdx.x;
-  if (idx >= 2 * p.half) return;
-  const int par = idx >= p.half;
-  idx -= par * p.half;
-  double phi[4], A[3], V[3];
-  for (int l = 0; l < 4; l++) phi[l] = (double)plane<T>(p, l, par)[idx];
-  for (int l = 0; l < 3; l++) A[l] = (double)plane<T>(p, 4 + l, par)[idx];
-  for (int l = 0; l < 3; l++) V[l] = (double)plane<T>(p, 7 + l, par)[idx];
-  rotate_site(phi, A, V, p.chi, p.tau_ideal);
-  for (int l = 0; l < 4; l++) p.scratch[(int64_t)(l * 2 + par) * p.half + idx] = phi[l];
+__device__ __forceinline__ void load_site(const ReplicaParams &p, int cur, int x, int y, int z,
+                                          double f[10]) {
+  const int par = (x + y + z) & 1;
+  const int64_t i = ((int64_t)z * p.N + y) * p.NH + (x >> 1);
+#pragma unroll
+  for (int c = 0; c < 10; c++) f[c] = (double)plane<T>(p, cur, c, par)[i];
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_ideal_kick_rot2(
-    const ReplicaParams *__restrict__ P, const int *__restrict__ blk_rep,
-    const int *__restrict__ blk_first) {
+__global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__restrict__ P,
+                                                        const ReplicaState *__restrict__ S,
+                                                        const int *__restrict__ blk_rep,
+                                                        const int *__restrict__ blk_first) {
+  __shared__ double lds[4][kIdealLds];
   const int r = blk_rep[blockIdx.x];
   const ReplicaParams p = P[r];
-  int64_t idx = (int64_t)(blockIdx.x - blk_first[r]) * kBlock + threadIdx.x;
-  if (idx >= 2 * p.half) return;
-  const int par = idx >= p.half;
-  idx -= par * p.half;
+  const int cur = S[r].cur, nxt = cur ^ 1;
   const int N = p.N, NH = p.NH;
-  const int xh = (int)(idx % NH);
-  const int64_t rowi = idx / NH;
-  const int y = (int)(rowi % N), z = (int)(rowi / N);
-  const int x = site_x(xh, y, z, par);
-  const int q = par ^ 1;
-  const int64_t row = rowi * NH;
-  const int64_t ixp = row + (wrapp(x + 1, N) >> 1), ixm = row + (wrapm(x - 1, N) >> 1);
-  const int64_t iyp = ((int64_t)z * N + wrapp(y + 1, N)) * NH + xh;
-  const int64_t iym = ((int64_t)z * N + wrapm(y - 1, N)) * NH + xh;
-  const int64_t izp = ((int64_t)wrapp(z + 1, N) * N + y) * NH + xh;
-  const int64_t izm = ((int64_t)wrapm(z - 1, N) * N + y) * NH + xh;
-  const double *sc = p.scratch;
-  double cp[4], xp[4], xm[4], yp[4], ym[4], zp[4], zm[4];
-  for (int l = 0; l < 4; l++) {
-    const double *own = sc + (int64_t)(l * 2 + par) * p.half;
-    const double *oth = sc + (int64_t)(l * 2 + q) * p.half;
-    cp[l] = own[idx];
-    xp[l] = oth[ixp]; xm[l] = oth[ixm];
-    yp[l] = oth[iyp]; ym[l] = oth[iym];
-    zp[l] = oth[izp]; zm[l] = oth[izm];
+  int TXH, TY, TZ;
+  ideal_tile(N, TXH, TY, TZ);
+  const int ntx = (NH + TXH - 1) / TXH, nty = (N + TY - 1) / TY;
+  int b = blockIdx.x - blk_first[r];
+  const int bx = b % ntx;
+  b /= ntx;
+  const int by = b % nty, bz = b / nty;
+  const int xh0 = bx * TXH, y0 = by * TY, z0 = bz * TZ;
+  const int tw = min(TXH, NH - xh0), th = min(TY, N - y0), tz = min(TZ, N - z0);
+  const int W = 2 * tw + 2;
+  const int t = threadIdx.x;
+  const int tx = t % TXH, ty = t / TXH;
+  const bool own = ty < th && tx < tw;
+  const int xh = xh0 + tx, y = y0 + ty;
+  // halo site of this thread (one-site ring around the tile, no corners)
+  const int nh = 2 * th + 4 * tw;
+  const bool halo = t < nh;
+  int hlx = 0, hly = 0;
+  if (t < th) { hlx = -1; hly = t; }
+  else if (t < 2 * th) { hlx = 2 * tw; hly = t - th; }
+  else if (t < 2 * th + 2 * tw) { hlx = t - 2 * th; hly = -1; }
+  else { hlx = t - 2 * th - 2 * tw; hly = th; }
+  const int hgx = (2 * xh0 + hlx + N) % N, hgy = (y0 + hly + N) % N;
+  const double h = 0.5 * (p.tau_ideal / p.chi);
+  const double axx = p.axx, H0 = p.H, dt = p.dt_ideal;
+
+  double prv[2][4], cen[2][4], nex[2][4];  // rotated phi, own sites, z-1, z, z+1
+  double chg[2][6], chn[2][6];             // raw charges of planes z and z+1
+  auto rotate_own = [&](int z, double out[2][4], double ch[2][6]) {
+    if (!own) return;
+    const int64_t i = ((int64_t)z * N + y) * NH + xh;
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+      const int par = (s + y + z) & 1;
+      double f[10];
+#pragma unroll
+      for (int c = 0; c < 10; c++) f[c] = (double)plane<T>(p, cur, c, par)[i];
+      rotate_site_fast(f, f + 4, f + 7, h);
+#pragma unroll
+      for (int c = 0; c < 4; c++) out[s][c] = f[c];
+#pragma unroll
+      for (int c = 0; c < 6; c++) ch[s][c] = f[4 + c];
+    }
+  };
+  auto fill_lds = [&](int z, const double o[2][4]) {
+    if (own) {
+#pragma unroll
+      for (int s = 0; s < 2; s++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) lds[c][(ty + 1) * W + (2 * tx + s + 1)] = o[s][c];
+    }
+    if (halo) {
+      double f[10];
+      load_site<T>(p, cur, hgx, hgy, z, f);
+      rotate_site_fast(f, f + 4, f + 7, h);
+#pragma unroll
+      for (int c = 0; c < 4; c++) lds[c][(hly + 1) * W + (hlx + 1)] = f[c];
+    }
+  };
+
+  double dummy[2][6];
+  rotate_own((z0 - 1 + N) % N, prv, dummy);
+  rotate_own(z0, cen, chg);
+  fill_lds(z0, cen);
+  for (int k = 0; k < tz; k++) {
+    const int z = z0 + k;
+    const int zn = (z + 1) % N;
+    rotate_own(zn, nex, chn);
+    __syncthreads();
+    if (own) {
+      const int64_t i = ((int64_t)z * N + y) * NH + xh;
+#pragma unroll
+      for (int s = 0; s < 2; s++) {
+        const int lx = 2 * tx + s + 1, ly = ty + 1;
+        double cp[4], xp[4], xm[4], yp[4], ym[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          cp[c] = cen[s][c];
+          xp[c] = lds[c][ly * W + lx + 1];
+          xm[c] = lds[c][ly * W + lx - 1];
+          yp[c] = lds[c][(ly + 1) * W + lx];
+          ym[c] = lds[c][(ly - 1) * W + lx];
+        }
+        const double *zp = nex[s], *zm = prv[s];
+        double A[3], V[3];
+#pragma unroll
+        for (int l = 0; l < 3; l++) {
+          A[l] = chg[s][l];
+          V[l] = chg[s][3 + l];
+        }
+#pragma unroll
+        for (int l = 0; l < 3; l++) {
+          const double advxx = (-xp[0] * cp[l + 1] + cp[0] * xp[l + 1] + cp[0] * xm[l + 1] -
+                                xm[0] * cp[l + 1]) * axx;
+          const double advyy = (-yp[0] * cp[l + 1] + cp[0] * yp[l + 1] + cp[0] * ym[l + 1] -
+                                ym[0] * cp[l + 1]) * axx;
+          const double advzz = (-zp[0] * cp[l + 1] + cp[0] * zp[l + 1] + cp[0] * zm[l + 1] -
+                                zm[0] * cp[l + 1]) * axx;
+          A[l] += dt * (advxx + advyy + advzz - H0 * cp[l + 1]);
+        }
+#pragma unroll
+        for (int s3 = 0; s3 < 3; s3++) {
+          const int s1 = (s3 + 1) % 3, s2 = (s3 + 2) % 3;
+          const double advxx = (xp[1 + s2] * cp[1 + s1] - cp[1 + s2] * xm[1 + s1] -
+                                xp[1 + s1] * cp[1 + s2] + cp[1 + s1] * xm[1 + s2]) * axx;
+          const double advyy = (yp[1 + s2] * cp[1 + s1] - cp[1 + s2] * ym[1 + s1] -
+                                yp[1 + s1] * cp[1 + s2] + cp[1 + s1] * ym[1 + s2]) * axx;
+          const double advzz = (zp[1 + s2] * cp[1 + s1] - cp[1 + s2] * zm[1 + s1] -
+                                zp[1 + s1] * cp[1 + s2] + cp[1 + s1] * zm[1 + s2]) * axx;
+          V[s3] += dt * (advxx + advyy + advzz);
+        }
+        rotate_site_fast(cp, A, V, h);
+        const int par = (s + y + z) & 1;
+#pragma unroll
+        for (int c = 0; c < 4; c++) plane<T>(p, nxt, c, par)[i] = (T)cp[c];
+#pragma unroll
+        for (int c = 0; c < 3; c++) plane<T>(p, nxt, 4 + c, par)[i] = (T)A[c];
+#pragma unroll
+        for (int c = 0; c < 3; c++) plane<T>(p, nxt, 7 + c, par)[i] = (T)V[c];
+      }
+    }
+    __syncthreads();
+    if (k + 1 < tz) fill_lds(zn, nex);
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        prv[s][c] = cen[s][c];
+        cen[s][c] = nex[s][c];
+      }
+#pragma unroll
+      for (int c = 0; c < 6; c++) chg[s][c] = chn[s][c];
+    }
   }
-  double A[3], V[3];
-  for (int l = 0; l < 3; l++) A[l] = (double)plane<T>(p, 4 + l, par)[idx];
-  for (int l = 0; l < 3; l++) V[l] = (double)plane<T>(p, 7 + l, par)[idx];
-  const double axx = p.axx, ayy = p.axx, azz = p.axx;
-  const double H0 = p.H, dt = p.dt_ideal;
-  for (int l = 0; l < 3; l++) {
-    const double advxx = (-xp[0] * cp[l + 1] + cp[0] * xp[l + 1] + cp[0] * xm[l + 1] -
-                          xm[0] * cp[l + 1]) * axx;
-    const double advyy = (-yp[0] * cp[l + 1] + cp[0] * yp[l + 1] + cp[0] * ym[l + 1] -
-                          ym[0] * cp[l + 1]) * ayy;
-    const double advzz = (-zp[0] * cp[l + 1] + cp[0] * zp[l + 1] + cp[0] * zm[l + 1] -
-                          zm[0] * cp[l + 1]) * azz;
-    A[l] += dt * (advxx + advyy + advzz - H0 * cp[l + 1]);
-  }
-  for (int s = 0; s < 3; s++) {
-    const int s1 = (s + 1) % 3, s2 = (s + 2) % 3;
-    const double advxx = (xp[1 + s2] * cp[1 + s1] - cp[1 + s2] * xm[1 + s1] -
-                          xp[1 + s1] * cp[1 + s2] + cp[1 + s1] * xm[1 + s2]) * axx;
-    const double advyy = (yp[1 + s2] * cp[1 + s1] - cp[1 + s2] * ym[1 + s1] -
-                          yp[1 + s1] * cp[1 + s2] + cp[1 + s1] * ym[1 + s2]) * ayy;
-    const double advzz = (zp[1 + s2] * cp[1 + s1] - cp[1 + s2] * zm[1 + s1] -
-                          zp[1 + s1] * cp[1 + s2] + cp[1 + s1] * zm[1 + s2]) * azz;
-    V[s] += dt * (advxx + advyy + advzz);
-  }
-  rotate_site(cp, A, V, p.chi, p.tau_ideal);
-  for (int l = 0; l < 4; l++) plane<T>(p, l, par)[idx] = (T)cp[l];
-  for (int l = 0; l < 3; l++) plane<T>(p, 4 + l, par)[idx] = (T)A[l];
-  for (int l = 0; l < 3; l++) plane<T>(p, 7 + l, par)[idx] = (T)V[l];
+}
+
+__global__ void k_flip(ReplicaState *S, int nrep) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < nrep) S[r].cur ^= 1;
 }
 
 // ------------------------------------------------------------------------
@@ -287,11 +378,13 @@ __global__ __launch_bounds__(kBlock) void k_ideal_kick_rot2(
 // partial sums of grad2, hEn and nab2.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_energy(const ReplicaParams *__restrict__ P,
+                                                   const ReplicaState *__restrict__ S,
                                                    const int *__restrict__ blk_rep,
                                                    const int *__restrict__ blk_first,
                                                    EnergySlot *__restrict__ slots) {
   const int r = blk_rep[blockIdx.x];
   const ReplicaParams p = P[r];
+  const int cur = S[r].cur;
   int64_t idx = (int64_t)(blockIdx.x - blk_first[r]) * kBlock + threadIdx.x;
   double g2 = 0., hE = 0., n2 = 0.;
   if (idx < 2 * p.half) {
@@ -307,23 +400,23 @@ __global__ __launch_bounds__(kBlock) void k_energy(const ReplicaParams *__restri
     const int64_t iyp = ((int64_t)z * N + wrapp(y + 1, N)) * NH + xh;
     const int64_t izp = ((int64_t)wrapp(z + 1, N) * N + y) * NH + xh;
     for (int s = 0; s < 4; s++) {
-      const double phimid = (double)plane<T>(p, s, par)[idx];
-      const T *o = plane<T>(p, s, q);
+      const double phimid = (double)plane<T>(p, cur, s, par)[idx];
+      const T *o = plane<T>(p, cur, s, q);
       double d;
       d = (double)o[izp] - phimid; g2 += d * d;
       d = (double)o[iyp] - phimid; g2 += d * d;
       d = (double)o[ixp] - phimid; g2 += d * d;
       if (s == 0) hE = phimid * p.H;
     }
-    for (int s = 0; s < 3; s++) { const double v = (double)plane<T>(p, 7 + s, par)[idx]; n2 += v * v; }
-    for (int s = 0; s < 3; s++) { const double a = (double)plane<T>(p, 4 + s, par)[idx]; n2 += a * a; }
+    for (int s = 0; s < 3; s++) { const double v = (double)plane<T>(p, cur, 7 + s, par)[idx]; n2 += v * v; }
+    for (int s = 0; s < 3; s++) { const double a = (double)plane<T>(p, cur, 4 + s, par)[idx]; n2 += a * a; }
   }
   KSum kg{g2, 0.}, kh{hE, 0.}, kn{n2, 0.};
   for (int o = 32; o > 0; o >>= 1) {
-    KSum t;
-    t.s = __shfl_xor(kg.s, o, 64); t.c = __shfl_xor(kg.c, o, 64); ksum_merge(kg, t);
-    t.s = __shfl_xor(kh.s, o, 64); t.c = __shfl_xor(kh.c, o, 64); ksum_merge(kh, t);
-    t.s = __shfl_xor(kn.s, o, 64); t.c = __shfl_xor(kn.c, o, 64); ksum_merge(kn, t);
+    KSum tt;
+    tt.s = __shfl_xor(kg.s, o, 64); tt.c = __shfl_xor(kg.c, o, 64); ksum_merge(kg, tt);
+    tt.s = __shfl_xor(kh.s, o, 64); tt.c = __shfl_xor(kh.c, o, 64); ksum_merge(kh, tt);
+    tt.s = __shfl_xor(kn.s, o, 64); tt.c = __shfl_xor(kn.c, o, 64); ksum_merge(kn, tt);
   }
   __shared__ KSum sh[3][kBlock / 64];
   const int w = threadIdx.x >> 6;
@@ -350,7 +443,6 @@ __global__ void k_energy_finish(const ReplicaParams *P, ReplicaState *S,
   const int b0 = blk_first[r];
   const int b1 = (r + 1 < nrep) ? blk_first[r + 1] : nblocks;
   KSum g{0, 0}, h{0, 0}, n{0, 0};
-  // lanes stride over the replica's blocks, then a fixed-order merge
   for (int b = b0 + (int)threadIdx.x; b < b1; b += 64) {
     ksum_merge(g, KSum{slots[b].g_s, slots[b].g_c});
     ksum_merge(h, KSum{slots[b].h_s, slots[b].h_c});
@@ -402,10 +494,12 @@ __global__ void k_set_time(ReplicaState *S, int nrep, int replica, double t) {
 }
 
 // ------------------------------------------------------------------------
-// o4fields [z][y][x][10] <-> red-black planes.
+// o4fields [z][y][x][10] <-> red-black planes of the live buffer.
 template <typename T>
-__global__ void k_aos_to_rb(const ReplicaParams *P, int replica, const double *aos) {
+__global__ void k_aos_to_rb(const ReplicaParams *P, const ReplicaState *S, int replica,
+                            const double *aos) {
   const ReplicaParams p = P[replica];
+  const int cur = S[replica].cur;
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t V = 2 * p.half;
   if (s >= V) return;
@@ -415,12 +509,14 @@ __global__ void k_aos_to_rb(const ReplicaParams *P, int replica, const double *a
   const int y = (int)(yz % N), z = (int)(yz / N);
   const int par = (x + y + z) & 1;
   const int64_t i = yz * p.NH + (x >> 1);
-  for (int c = 0; c < 10; c++) plane<T>(p, c, par)[i] = (T)aos[s * 10 + c];
+  for (int c = 0; c < 10; c++) plane<T>(p, cur, c, par)[i] = (T)aos[s * 10 + c];
 }
 
 template <typename T>
-__global__ void k_rb_to_aos(const ReplicaParams *P, int replica, double *aos) {
+__global__ void k_rb_to_aos(const ReplicaParams *P, const ReplicaState *S, int replica,
+                            double *aos) {
   const ReplicaParams p = P[replica];
+  const int cur = S[replica].cur;
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t V = 2 * p.half;
   if (s >= V) return;
@@ -430,7 +526,7 @@ __global__ void k_rb_to_aos(const ReplicaParams *P, int replica, double *aos) {
   const int y = (int)(yz % N), z = (int)(yz / N);
   const int par = (x + y + z) & 1;
   const int64_t i = yz * p.NH + (x >> 1);
-  for (int c = 0; c < 10; c++) aos[s * 10 + c] = (double)plane<T>(p, c, par)[i];
+  for (int c = 0; c < 10; c++) aos[s * 10 + c] = (double)plane<T>(p, cur, c, par)[i];
 }
 
 // ------------------------------------------------------------------------
@@ -445,10 +541,10 @@ __device__ __forceinline__ void contract_rho(const double n[4], const double rho
 }
 
 template <typename T>
-__device__ __forceinline__ void site_obs(const ReplicaParams &p, int par, int64_t i,
+__device__ __forceinline__ void site_obs(const ReplicaParams &p, int cur, int par, int64_t i,
                                          double o[25]) {
   double f[10];
-  for (int c = 0; c < 10; c++) f[c] = (double)plane<T>(p, c, par)[i];
+  for (int c = 0; c < 10; c++) f[c] = (double)plane<T>(p, cur, c, par)[i];
   double phi2 = 0.0;
   for (int l = 0; l < 4; l++) phi2 += f[l] * f[l];
   for (int c = 0; c < 10; c++) o[c] = f[c];
@@ -477,9 +573,11 @@ __device__ __forceinline__ void site_obs(const ReplicaParams &p, int par, int64_
 // One thread per (x, z): compensated sum over y -> partXZ[o][z][x];
 // one thread per (x, y): compensated sum over z -> partXY[o][y][x].
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_measure_cols(const ReplicaParams *P, int replica,
+__global__ __launch_bounds__(kBlock) void k_measure_cols(const ReplicaParams *P,
+                                                         const ReplicaState *S, int replica,
                                                          int along_z, double *part) {
   const ReplicaParams p = P[replica];
+  const int cur = S[replica].cur;
   const int N = p.N;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)N * N) return;
@@ -492,7 +590,7 @@ __global__ __launch_bounds__(kBlock) void k_measure_cols(const ReplicaParams *P,
     const int par = (x + y + z) & 1;
     const int64_t i = ((int64_t)z * N + y) * p.NH + (x >> 1);
     double o[25];
-    site_obs<T>(p, par, i, o);
+    site_obs<T>(p, cur, par, i, o);
     for (int k = 0; k < 25; k++) ksum_add(acc[k], o[k]);
   }
   for (int k = 0; k < 25; k++) part[((int64_t)k * N + w) * N + x] = ksum_value(acc[k]);
@@ -549,30 +647,24 @@ void launch_charge(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaS
                                                       sub, ctr_off, mon);
 }
 
-void launch_ideal_rot1(bool fp32, const Plan &plan, const ReplicaParams *P, hipStream_t st) {
+void launch_ideal(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
+                  int nrep, hipStream_t st) {
   if (fp32)
-    k_ideal_rot1<float><<<plan.nblocks, kBlock, 0, st>>>(P, plan.d_blk_rep, plan.d_blk_first);
+    k_ideal_fused<float><<<plan.nblocks, kBlock, 0, st>>>(P, S, plan.d_blk_rep,
+                                                          plan.d_blk_first);
   else
-    k_ideal_rot1<double><<<plan.nblocks, kBlock, 0, st>>>(P, plan.d_blk_rep, plan.d_blk_first);
+    k_ideal_fused<double><<<plan.nblocks, kBlock, 0, st>>>(P, S, plan.d_blk_rep,
+                                                           plan.d_blk_first);
+  k_flip<<<nblk(nrep, 64), 64, 0, st>>>(S, nrep);
 }
 
-void launch_ideal_kick_rot2(bool fp32, const Plan &plan, const ReplicaParams *P,
-                            hipStream_t st) {
+void launch_energy(bool fp32, const Plan &plan, const ReplicaParams *P,
+                   const ReplicaState *S, EnergySlot *slots, hipStream_t st) {
   if (fp32)
-    k_ideal_kick_rot2<float><<<plan.nblocks, kBlock, 0, st>>>(P, plan.d_blk_rep,
-                                                              plan.d_blk_first);
-  else
-    k_ideal_kick_rot2<double><<<plan.nblocks, kBlock, 0, st>>>(P, plan.d_blk_rep,
-                                                               plan.d_blk_first);
-}
-
-void launch_energy(bool fp32, const Plan &plan, const ReplicaParams *P, EnergySlot *slots,
-                   hipStream_t st) {
-  if (fp32)
-    k_energy<float><<<plan.nblocks, kBlock, 0, st>>>(P, plan.d_blk_rep, plan.d_blk_first,
+    k_energy<float><<<plan.nblocks, kBlock, 0, st>>>(P, S, plan.d_blk_rep, plan.d_blk_first,
                                                      slots);
   else
-    k_energy<double><<<plan.nblocks, kBlock, 0, st>>>(P, plan.d_blk_rep, plan.d_blk_first,
+    k_energy<double><<<plan.nblocks, kBlock, 0, st>>>(P, S, plan.d_blk_rep, plan.d_blk_first,
                                                       slots);
 }
 
@@ -592,29 +684,30 @@ void launch_set_time(int nrep, int replica, ReplicaState *S, double t, hipStream
   k_set_time<<<nblk(nrep, 64), 64, 0, st>>>(S, nrep, replica, t);
 }
 
-void launch_aos_to_rb(bool fp32, const ReplicaParams *P, int replica, int N,
-                      const double *aos, hipStream_t st) {
+void launch_aos_to_rb(bool fp32, const ReplicaParams *P, const ReplicaState *S, int replica,
+                      int N, const double *aos, hipStream_t st) {
   const int64_t V = (int64_t)N * N * N;
-  if (fp32) k_aos_to_rb<float><<<nblk(V, 256), 256, 0, st>>>(P, replica, aos);
-  else k_aos_to_rb<double><<<nblk(V, 256), 256, 0, st>>>(P, replica, aos);
+  if (fp32) k_aos_to_rb<float><<<nblk(V, 256), 256, 0, st>>>(P, S, replica, aos);
+  else k_aos_to_rb<double><<<nblk(V, 256), 256, 0, st>>>(P, S, replica, aos);
 }
 
-void launch_rb_to_aos(bool fp32, const ReplicaParams *P, int replica, int N, double *aos,
-                      hipStream_t st) {
+void launch_rb_to_aos(bool fp32, const ReplicaParams *P, const ReplicaState *S, int replica,
+                      int N, double *aos, hipStream_t st) {
   const int64_t V = (int64_t)N * N * N;
-  if (fp32) k_rb_to_aos<float><<<nblk(V, 256), 256, 0, st>>>(P, replica, aos);
-  else k_rb_to_aos<double><<<nblk(V, 256), 256, 0, st>>>(P, replica, aos);
+  if (fp32) k_rb_to_aos<float><<<nblk(V, 256), 256, 0, st>>>(P, S, replica, aos);
+  else k_rb_to_aos<double><<<nblk(V, 256), 256, 0, st>>>(P, S, replica, aos);
 }
 
-void launch_measure(bool fp32, const ReplicaParams *P, int replica, int N, double *partXZ,
-                    double *partXY, double *wall, double *wallphase, hipStream_t st) {
+void launch_measure(bool fp32, const ReplicaParams *P, const ReplicaState *S, int replica,
+                    int N, double *partXZ, double *partXY, double *wall, double *wallphase,
+                    hipStream_t st) {
   const int64_t cols = (int64_t)N * N;
   if (fp32) {
-    k_measure_cols<float><<<nblk(cols, kBlock), kBlock, 0, st>>>(P, replica, 0, partXZ);
-    k_measure_cols<float><<<nblk(cols, kBlock), kBlock, 0, st>>>(P, replica, 1, partXY);
+    k_measure_cols<float><<<nblk(cols, kBlock), kBlock, 0, st>>>(P, S, replica, 0, partXZ);
+    k_measure_cols<float><<<nblk(cols, kBlock), kBlock, 0, st>>>(P, S, replica, 1, partXY);
   } else {
-    k_measure_cols<double><<<nblk(cols, kBlock), kBlock, 0, st>>>(P, replica, 0, partXZ);
-    k_measure_cols<double><<<nblk(cols, kBlock), kBlock, 0, st>>>(P, replica, 1, partXY);
+    k_measure_cols<double><<<nblk(cols, kBlock), kBlock, 0, st>>>(P, S, replica, 0, partXZ);
+    k_measure_cols<double><<<nblk(cols, kBlock), kBlock, 0, st>>>(P, S, replica, 1, partXY);
   }
   k_measure_reduce<<<nblk(3 * 25 * N, 256), 256, 0, st>>>(N, partXZ, partXY, wall, wallphase);
 }
