@@ -148,6 +148,27 @@ enum { LGV_CLASS_IDEAL = 0, LGV_CLASS_PHI = 1, LGV_CLASS_CHARGE = 2, LGV_CLASS_E
 int lgv_timing_enable(lgv_ctx *ctx, int enable);
 int lgv_timing_get(lgv_ctx *ctx, double ms[LGV_NCLASS], long long launches[LGV_NCLASS]);
 
+/* Measurer::measure in full (measurer.h:169-179, measurer.cxx:10-317):
+ * walls, OAverage, the r2c transforms / N of every wall row (FFTW forward
+ * sign, k = 0..N/2; computed on the device) and rotate_to_vev of the
+ * k-space walls.  Any pointer may be NULL.  Shapes, d = x, y, z:
+ *   wall[3][11][N]  wallphase[3][14][N]  oavg[11]
+ *   wall_k[3][11][N/2+1][2]  wall_k_rotated[3][14][N/2+1][2]
+ *   wallphase_k[3][14][N/2+1][2]
+ * (the HDF5 rows of measurer_output.cxx:71-130). */
+typedef struct {
+  double *wall, *wallphase, *oavg;
+  double *wall_k, *wall_k_rotated, *wallphase_k;
+} lgv_measurement;
+int lgv_measure_full(lgv_ctx *ctx, int replica, lgv_measurement *m);
+
+/* ModelACoefficients::mass0 / dmassdt (ModelA.h:73-74), changed at run
+ * time by thermalize_event's quench mode (SuperPions.cxx:31-75).
+ * replica < 0 sets every replica. */
+int lgv_set_mass(lgv_ctx *ctx, int replica, double mass0, double dmassdt);
+/* Zeroes every o4_stepper_monitor counter (a fresh stepper object). */
+int lgv_monitor_reset(lgv_ctx *ctx);
+
 int lgv_synchronize(lgv_ctx *ctx);
 /* Device address of a replica's field storage (10 component planes, each
  * split into even and odd half-lattices) and its element size in bytes. */

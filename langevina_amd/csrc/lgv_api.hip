@@ -1,7 +1,10 @@
 // lgv_api.hip -- the C ABI of include/lgv.h: context, replica storage,
 // composition of PV2HBSplit::step from the kernels, host-side global engine
 // (initial charges), monitors and measurement.
+#include <cfloat>
 #include <cmath>
+#include <complex>
+#include <limits>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -93,6 +96,7 @@ struct lgv_ctx {
   double *d_aos = nullptr;
   int maxN = 0;
   double *d_partXZ = nullptr, *d_partXY = nullptr, *d_wall = nullptr, *d_wallphase = nullptr;
+  double *d_wallk = nullptr, *d_wallphasek = nullptr;
   bool use_graph = false;
   hipGraphExec_t graph_exec = nullptr;
   double graph_dt = NAN;
@@ -359,6 +363,8 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
   HIPCHECK(hipMalloc(&c->d_partXY, (int64_t)25 * c->maxN * c->maxN * sizeof(double)));
   HIPCHECK(hipMalloc(&c->d_wall, 3 * 11 * c->maxN * sizeof(double)));
   HIPCHECK(hipMalloc(&c->d_wallphase, 3 * 14 * c->maxN * sizeof(double)));
+  HIPCHECK(hipMalloc(&c->d_wallk, 3 * 11 * (c->maxN / 2 + 1) * 2 * sizeof(double)));
+  HIPCHECK(hipMalloc(&c->d_wallphasek, 3 * 14 * (c->maxN / 2 + 1) * 2 * sizeof(double)));
   HIPCHECK(hipDeviceSynchronize());
   *out = c.release();
   return LGV_OK;
@@ -386,6 +392,8 @@ int lgv_destroy(lgv_ctx *c) {
   hipFree(c->d_partXY);
   hipFree(c->d_wall);
   hipFree(c->d_wallphase);
+  hipFree(c->d_wallk);
+  hipFree(c->d_wallphasek);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return LGV_OK;
@@ -669,6 +677,120 @@ int lgv_timing_get(lgv_ctx *c, double ms[LGV_NCLASS], long long launches[LGV_NCL
     ms[k] = c->t_ms[k];
     launches[k] = c->t_launch[k];
   }
+  return LGV_OK;
+}
+
+// rotate_to_vev (measurer.cxx:231-286): rotates the k-space walls into the
+// frame of the k = 0 phi direction.  wk[11][nk][2] -> rot[14][nk][2].
+static void rotate_to_vev(int nk, const double *wk, double *rot) {
+  using cd = std::complex<double>;
+  auto W = [&](int a, int k) { return cd(wk[(a * nk + k) * 2], wk[(a * nk + k) * 2 + 1]); };
+  auto contract = [](const cd *n, const cd *rho, cd *out) {
+    out[0] = -(n[1] * rho[0]) - n[2] * rho[1] - n[3] * rho[2];
+    out[1] = n[0] * rho[0] + n[3] * rho[4] - n[2] * rho[5];
+    out[2] = n[0] * rho[1] - n[3] * rho[3] + n[1] * rho[5];
+    out[3] = n[0] * rho[2] + n[2] * rho[3] - n[1] * rho[4];
+  };
+  cd n[4];
+  double norm = 0.;
+  for (int a = 0; a < 4; a++) {
+    n[a] = W(a, 0);
+    norm += std::norm(n[a]);
+  }
+  norm = std::sqrt(norm);
+  if (norm < std::numeric_limits<double>::min()) {
+    n[0] = 1; n[1] = n[2] = n[3] = 0;
+  } else {
+    for (int a = 0; a < 4; a++) n[a] /= norm;
+  }
+  for (int k = 0; k < nk; k++) {
+    cd phi[4], rho[6], drho[6], phir[4], A[4], V[4];
+    for (int a = 0; a < 4; a++) phi[a] = W(a, k);
+    for (int ab = 0; ab < 6; ab++) rho[ab] = W(4 + ab, k);
+    for (int l = 0; l < 3; l++) { drho[l] = rho[l + 3]; drho[l + 3] = rho[l]; }
+    const cd phi2k = W(10, k);
+    cd phis(0.);
+    for (int a = 0; a < 4; a++) phis += n[a] * phi[a];
+    for (int a = 0; a < 4; a++) phir[a] = phi[a] - n[a] * phis;
+    contract(n, rho, A);
+    contract(n, drho, V);
+    auto put = [&](int a, cd v) {
+      rot[(a * nk + k) * 2] = v.real();
+      rot[(a * nk + k) * 2 + 1] = v.imag();
+    };
+    put(0, phis);
+    for (int a = 0; a < 4; a++) {
+      put(1 + a, phir[a]);
+      put(5 + a, A[a]);
+      put(9 + a, V[a]);
+    }
+    put(13, phi2k);
+  }
+}
+
+int lgv_measure_full(lgv_ctx *c, int r, lgv_measurement *m) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  if (!m) return fail(LGV_ERR_ARG, "null measurement");
+  HIPCHECK(hipSetDevice(c->device));
+  const int N = c->hp[r].N, nk = N / 2 + 1;
+  std::vector<double> wall(3 * 11 * N), wallphase(3 * 14 * N), oavg(11);
+  launch_measure(c->fp32, c->dP, c->dS, r, N, c->d_partXZ, c->d_partXY, c->d_wall,
+                 c->d_wallphase, c->stream);
+  launch_wall_dft(N, c->d_wall, c->d_wallphase, c->d_wallk, c->d_wallphasek, c->stream);
+  if ((rc = check_launch())) return rc;
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  HIPCHECK(hipMemcpy(wall.data(), c->d_wall, wall.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(wallphase.data(), c->d_wallphase, wallphase.size() * sizeof(double),
+                     hipMemcpyDeviceToHost));
+  std::vector<double> wk(3 * 11 * nk * 2), wpk(3 * 14 * nk * 2);
+  HIPCHECK(hipMemcpy(wk.data(), c->d_wallk, wk.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(wpk.data(), c->d_wallphasek, wpk.size() * sizeof(double),
+                     hipMemcpyDeviceToHost));
+  for (int l = 0; l < 11; l++) {
+    double s = 0.;
+    for (int i = 0; i < N; i++) s += wall[l * N + i];
+    oavg[l] = s / (double)N;
+  }
+  if (m->wall) memcpy(m->wall, wall.data(), wall.size() * sizeof(double));
+  if (m->wallphase) memcpy(m->wallphase, wallphase.data(), wallphase.size() * sizeof(double));
+  if (m->oavg) memcpy(m->oavg, oavg.data(), 11 * sizeof(double));
+  if (m->wall_k) memcpy(m->wall_k, wk.data(), wk.size() * sizeof(double));
+  if (m->wallphase_k) memcpy(m->wallphase_k, wpk.data(), wpk.size() * sizeof(double));
+  if (m->wall_k_rotated)
+    for (int d = 0; d < 3; d++)
+      rotate_to_vev(nk, wk.data() + (size_t)d * 11 * nk * 2,
+                    m->wall_k_rotated + (size_t)d * 14 * nk * 2);
+  return LGV_OK;
+}
+
+int lgv_set_mass(lgv_ctx *c, int r, double mass0, double dmassdt) {
+  if (!c) return fail(LGV_ERR_ARG, "null context");
+  if (r >= c->nrep) return fail(LGV_ERR_RANGE, "replica out of range");
+  HIPCHECK(hipSetDevice(c->device));
+  for (int i = 0; i < c->nrep; i++) {
+    if (r >= 0 && i != r) continue;
+    c->mp[i].mass0 = c->hp[i].mass0 = mass0;
+    c->mp[i].dmassdt = c->hp[i].dmassdt = dmassdt;
+  }
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  HIPCHECK(hipMemcpy(c->dP, c->hp.data(), c->nrep * sizeof(ReplicaParams),
+                     hipMemcpyHostToDevice));
+  if (c->graph_exec) {
+    HIPCHECK(hipGraphExecDestroy(c->graph_exec));
+    c->graph_exec = nullptr;
+  }
+  return LGV_OK;
+}
+
+int lgv_monitor_reset(lgv_ctx *c) {
+  if (!c) return fail(LGV_ERR_ARG, "null context");
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  HIPCHECK(hipMemset(c->mon_phi, 0, c->plan_half.nblocks * sizeof(MonSlot)));
+  HIPCHECK(hipMemset(c->mon_charge, 0, c->plan_half.nblocks * sizeof(MonSlot)));
+  HIPCHECK(hipMemset(c->mon_ideal, 0, c->nrep * sizeof(MonSlot)));
+  HIPCHECK(hipDeviceSynchronize());
   return LGV_OK;
 }
 

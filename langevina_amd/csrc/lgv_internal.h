@@ -86,6 +86,8 @@ void launch_aos_to_rb(bool fp32, const ReplicaParams *P, const ReplicaState *S, 
                       int N, const double *aos, hipStream_t st);
 void launch_rb_to_aos(bool fp32, const ReplicaParams *P, const ReplicaState *S, int replica,
                       int N, double *aos, hipStream_t st);
+void launch_wall_dft(int N, const double *wall, const double *wallphase, double *wall_k,
+                     double *wallphase_k, hipStream_t st);
 void launch_measure(bool fp32, const ReplicaParams *P, const ReplicaState *S, int replica,
                     int N, double *partXZ, double *partXY, double *wall, double *wallphase,
                     hipStream_t st);

@@ -618,7 +618,41 @@ __global__ void k_measure_reduce(int N, const double *partXZ, const double *part
   else wallphase[((int64_t)d * 14 + (k - 11)) * N + n] = val;
 }
 
+// measurer_fft::execute_row (measurer.h:33-41): FFTW's r2c forward
+// transform divided by N, out[k] = (1/N) sum_x in[x] e^{-2 pi i k x / N},
+// k = 0..N/2, as a direct DFT with exactly reduced twiddles and compensated
+// sums.  One thread per (row, k); rows = 3 x 25 walls.
+__global__ void k_wall_dft(int N, const double *wall, const double *wallphase, double *wall_k,
+                           double *wallphase_k) {
+  const int nk = N / 2 + 1;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * 25 * nk) return;
+  const int k = t % nk;
+  const int row = t / nk;  // d*25 + obs
+  const int d = row / 25, o = row % 25;
+  const double *in = o < 11 ? wall + ((int64_t)d * 11 + o) * N
+                            : wallphase + ((int64_t)d * 14 + (o - 11)) * N;
+  KSum re{0., 0.}, im{0., 0.};
+  for (int x = 0; x < N; x++) {
+    const int m = (int)(((int64_t)k * x) % N);
+    double s, c;
+    sincospi(2.0 * m / N, &s, &c);
+    ksum_add(re, in[x] * c);
+    ksum_add(im, -(in[x] * s));
+  }
+  double *out = o < 11 ? wall_k + (((int64_t)d * 11 + o) * nk + k) * 2
+                       : wallphase_k + (((int64_t)d * 14 + (o - 11)) * nk + k) * 2;
+  out[0] = ksum_value(re) / N;
+  out[1] = ksum_value(im) / N;
+}
+
 // ======================================================================== launchers
+
+void launch_wall_dft(int N, const double *wall, const double *wallphase, double *wall_k,
+                     double *wallphase_k, hipStream_t st) {
+  const int n = 3 * 25 * (N / 2 + 1);
+  k_wall_dft<<<(n + 127) / 128, 128, 0, st>>>(N, wall, wallphase, wall_k, wallphase_k);
+}
 
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
