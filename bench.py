@@ -51,6 +51,95 @@ def cpu_baseline(seconds_target=12.0):
                       f"lattice ({el:.1f} s, 1 thread)"}
 
 
+def batched_replicas(L):
+    """96 independent replicas: 48 x 32^3, 32 x 48^3, 16 x 64^3; seed 1000+r;
+    H_r uniform in [0.003, 0.03] from a xoroshiro128+ stream seeded with
+    12345 by the reference's seed formula; m^2 = -4.813."""
+    from langevina_amd.lgv import model_params
+    sizes = [32] * 48 + [48] * 32 + [64] * 16
+    s0 = (0x01d353e5f3993bb0 + 12345) % 2**64
+    s1 = (0x7b9c0df6cb193b20 * 12346) % 2**64
+    reps = []
+    for r, N in enumerate(sizes):
+        res = (s0 + s1) % 2**64  # xoroshiro128+ output
+        t = s1 ^ s0
+        s0 = (((s0 << 55) | (s0 >> 9)) % 2**64) ^ t ^ ((t << 14) % 2**64)
+        s1 = ((t << 36) | (t >> 28)) % 2**64
+        u = float(res) / 2**64
+        H = 0.003 + (0.03 - 0.003) * u
+        reps.append(model_params(N, seed=1000 + r, **dict(PARAMS, H=H)))
+    return reps
+
+
+def bench_batched(args):
+    import langevina_amd as L
+    from langevina_amd.autotune import autotune_ideal
+    reps = batched_replicas(L)
+    sites = sum(p.NX ** 3 for p in reps)
+    prec = L.FP32 if args.precision == "fp32" else L.FP64
+    ctx = L.Context(reps, precision=prec)
+    for r in range(len(reps)):
+        ctx.init_gaussian_charges(r)
+    tuned = autotune_ideal(ctx, DT) if not args.no_autotune else {}
+    ctx.timing_enable(True)  # kernel breakdown from a short direct-launch pass
+    ctx.run(DT, 5)
+    tm = ctx.timing()
+    ctx.timing_enable(False)
+    ctx.use_graph(True)
+    ctx.run(DT, args.warmup)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.run(DT, args.steps)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    value = sites * args.steps / el
+    ctx.close()
+    # the same replicas one context (and one graph) at a time
+    seq_el = 0.0
+    for p in reps:
+        c1 = L.Context(p, precision=prec)
+        c1.init_gaussian_charges()
+        if tuned:
+            from langevina_amd.autotune import set_ideal_tile
+            cap, tz = tuned[p.NX][:2]
+            set_ideal_tile(c1, 0, cap, tz)
+        c1.use_graph(True)
+        c1.run(DT, 2)
+        c1.synchronize()
+        t0 = time.perf_counter()
+        c1.run(DT, args.steps)
+        c1.synchronize()
+        seq_el += time.perf_counter() - t0
+        c1.close()
+    seq_value = sites * args.steps / seq_el
+    esz = 8 if prec == L.FP64 else 4
+    dom = max(("ideal", "phi", "charge"), key=lambda k: tm[k][0])
+    ms, launches = tm[dom]
+    per_launch_s = ms * 1e-3 / max(launches, 1)
+    bytes_per_launch = ALG_BYTES[dom] * sites * esz // 8
+    achieved = bytes_per_launch / per_launch_s
+    out = {
+        "metric": "site-updates/sec (ModelG 256^3 FP64 vs FP32/batched) and % HBM roofline, 1 GPU",
+        "value": value, "unit": "site-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": el * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-storage/f64-arith" if prec == L.FP32 else "f64",
+        "data": "synthetic (phi=0, Gaussian charges per replica)",
+        "config": {"workload": "batched ModelG: 96 replicas (48x32^3, 32x48^3, 16x64^3), "
+                               "one HIP graph per step",
+                   "sites": sites, "dt": DT, "parallelism": "1 GPU",
+                   "ideal_tiles": {str(k): list(v[:2]) for k, v in tuned.items()}},
+        "one_at_a_time": {"value": seq_value, "ms_per_step": seq_el * 1e3 / args.steps},
+        "batched_over_one_at_a_time": value / seq_value,
+        "step_roofline_frac": value * ALG_BYTES_PER_SITE_UPDATE * esz / 8 / HBM_PEAK,
+        "roofline": {"bound": "hbm", "kernel": {"ideal": "k_ideal_fused", "phi": "k_hb_phi",
+                                                 "charge": "k_charge"}[dom],
+                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,12 +148,18 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--graph", action="store_true", help="replay a HIP graph per step")
+    ap.add_argument("--batched", action="store_true",
+                    help="96 mixed-size replicas in one context (use with --precision fp32)")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the lattice and every replica "
               "live on a single MI355X); --gpus must be 1", file=sys.stderr)
         sys.exit(2)
+    if args.batched:
+        bench_batched(args)
+        return
 
     import numpy as np
     import langevina_amd as L

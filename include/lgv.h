@@ -169,6 +169,11 @@ int lgv_set_mass(lgv_ctx *ctx, int replica, double mass0, double dmassdt);
 /* Zeroes every o4_stepper_monitor counter (a fresh stepper object). */
 int lgv_monitor_reset(lgv_ctx *ctx);
 
+/* Tile of the fused ideal-step kernel for one replica (replica < 0: all):
+ * at most txh_cap half-columns (2..64) in x, 256/txh rows in y, tz planes
+ * per z-chunk.  Results do not depend on it; the autotuner picks it. */
+int lgv_set_ideal_tile(lgv_ctx *ctx, int replica, int txh_cap, int tz);
+
 int lgv_synchronize(lgv_ctx *ctx);
 /* Device address of a replica's field storage (10 component planes, each
  * split into even and odd half-lattices) and its element size in bytes. */

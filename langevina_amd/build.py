@@ -36,6 +36,30 @@ def _compile(src):
     return obj
 
 
+BIN = os.path.join(PKG, "bin")
+DRIVER = os.path.join(BIN, "lgv_superpions")
+HDF5_INCLUDE = os.environ.get("LGV_HDF5_INCLUDE", "/opt/conda/include")
+
+
+def build_driver(verbose=False):
+    """The C++ run driver (host code only): HDF5 headers at compile time,
+    libhdf5 dlopen'ed at run time, liblgv_hip.so found through $ORIGIN."""
+    os.makedirs(BIN, exist_ok=True)
+    tmp = DRIVER + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall",
+           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(PKG, "host"),
+           "-idirafter", HDF5_INCLUDE,
+           os.path.join(PKG, "host", "superpions.cpp"), "-o", tmp,
+           "-L", PKG, "-llgv_hip", "-Wl,-rpath,$ORIGIN/..", "-ldl"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, DRIVER)
+    if verbose:
+        print("built", DRIVER)
+    return DRIVER
+
+
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
@@ -48,6 +72,10 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    if os.path.exists(os.path.join(HDF5_INCLUDE, "hdf5.h")):
+        build_driver(verbose)
+    elif verbose:
+        print("hdf5.h not found: run driver not built")
     return LIB
 
 

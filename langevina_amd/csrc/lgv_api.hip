@@ -128,6 +128,28 @@ int build_plan(Plan &plan, const std::vector<int64_t> &work) {
   return LGV_OK;
 }
 
+// Blocks of the fused ideal kernel: one per (x-tile, y-tile, z-chunk) of
+// every replica, with that replica's tile shape.
+int build_ideal_plan(lgv_ctx *c) {
+  std::vector<int64_t> work;
+  for (int r = 0; r < c->nrep; r++) {
+    const ReplicaParams &p = c->hp[r];
+    int TXH, TY, TZ;
+    ideal_tile(p.N, p.tile_cap, p.tile_tz, TXH, TY, TZ);
+    if ((TY + 2) * (2 * TXH + 2) > kIdealLds)
+      return fail(LGV_ERR_ARG, "replica %d: ideal tile %dx%d exceeds the LDS plane", r, TXH, TY);
+    work.push_back((int64_t)((p.NH + TXH - 1) / TXH) * ((p.N + TY - 1) / TY) *
+                   ((p.N + TZ - 1) / TZ) * kBlock);
+  }
+  if (c->plan_ideal.d_blk_rep) {
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    HIPCHECK(hipFree(c->plan_ideal.d_blk_rep));
+    HIPCHECK(hipFree(c->plan_ideal.d_blk_first));
+    c->plan_ideal = Plan{};
+  }
+  return build_plan(c->plan_ideal, work);
+}
+
 // Step coefficients with the reference's expressions:
 //   EulerLangevinHB  rdtg = sqrt(2.*dt*gamma)          Stepper.cxx:301
 //   ModelGChargeHB   rdtsigma = sqrt(2.*dt*sigma())    Stepper.cxx:456
@@ -221,12 +243,14 @@ void enqueue_hb(lgv_ctx *c, int off) {
   }
 }
 
+// ModelGChargeHB::step: the shuffle, then per direction both parities in
+// one pass (counters off+1+2*ixyz and off+2+2*ixyz, as the unfused order).
 void enqueue_charge(lgv_ctx *c, int off) {
   launch_shuffle(c->nrep, c->dS, c->dP, off, c->stream);
-  for (int sub = 0; sub < 6; sub++) {
+  for (int ixyz = 0; ixyz < 3; ixyz++) {
     Timed tm(c, LGV_CLASS_CHARGE);
-    launch_charge(c->fp32, c->plan_half, c->dP, c->dS, sub, off + 1 + sub, c->mon_charge,
-                  c->stream);
+    launch_charge_dir(c->fp32, c->plan_half, c->dP, c->dS, c->nrep, ixyz, off + 1 + 2 * ixyz,
+                      c->mon_charge, c->stream);
   }
 }
 
@@ -299,7 +323,7 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
   c->hp.resize(c->nrep);
   c->noise.resize(c->nrep);
   const size_t esz = c->fp32 ? sizeof(float) : sizeof(double);
-  std::vector<int64_t> whalf, wfull, wideal;
+  std::vector<int64_t> whalf, wfull;
   std::vector<ReplicaState> hs(c->nrep);
   for (int r = 0; r < c->nrep; r++) {
     const lgv_model_params &m = c->mp[r];
@@ -314,12 +338,8 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
       c->fields.push_back(field);
       p.fields[b] = field;
     }
-    int TXH, TY, TZ;
-    ideal_tile(N, TXH, TY, TZ);
-    if ((TY + 2) * (2 * TXH + 2) > kIdealLds)
-      return fail(LGV_ERR_ARG, "replica %d: NX=%d gives an ideal tile beyond LDS", r, N);
-    wideal.push_back((int64_t)((N / 2 + TXH - 1) / TXH) * ((N + TY - 1) / TY) *
-                     ((N + TZ - 1) / TZ) * kBlock);
+    p.tile_cap = 32;
+    p.tile_tz = 16;
     p.N = N;
     p.NH = N / 2;
     p.half = half;
@@ -349,7 +369,7 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
   int rc;
   if ((rc = build_plan(c->plan_half, whalf)) != LGV_OK) return rc;
   if ((rc = build_plan(c->plan_full, wfull)) != LGV_OK) return rc;
-  if ((rc = build_plan(c->plan_ideal, wideal)) != LGV_OK) return rc;
+  if ((rc = build_ideal_plan(c.get())) != LGV_OK) return rc;
   HIPCHECK(hipMalloc(&c->mon_phi, c->plan_half.nblocks * sizeof(MonSlot)));
   HIPCHECK(hipMemset(c->mon_phi, 0, c->plan_half.nblocks * sizeof(MonSlot)));
   HIPCHECK(hipMalloc(&c->mon_charge, c->plan_half.nblocks * sizeof(MonSlot)));
@@ -772,6 +792,34 @@ int lgv_set_mass(lgv_ctx *c, int r, double mass0, double dmassdt) {
     if (r >= 0 && i != r) continue;
     c->mp[i].mass0 = c->hp[i].mass0 = mass0;
     c->mp[i].dmassdt = c->hp[i].dmassdt = dmassdt;
+  }
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  HIPCHECK(hipMemcpy(c->dP, c->hp.data(), c->nrep * sizeof(ReplicaParams),
+                     hipMemcpyHostToDevice));
+  if (c->graph_exec) {
+    HIPCHECK(hipGraphExecDestroy(c->graph_exec));
+    c->graph_exec = nullptr;
+  }
+  return LGV_OK;
+}
+
+int lgv_set_ideal_tile(lgv_ctx *c, int r, int txh_cap, int tz) {
+  if (!c) return fail(LGV_ERR_ARG, "null context");
+  if (r >= c->nrep) return fail(LGV_ERR_RANGE, "replica out of range");
+  if (txh_cap < 2 || txh_cap > 64 || tz < 1 || tz > 1024)
+    return fail(LGV_ERR_ARG, "tile cap %d / tz %d out of range", txh_cap, tz);
+  HIPCHECK(hipSetDevice(c->device));
+  std::vector<ReplicaParams> old = c->hp;
+  for (int i = 0; i < c->nrep; i++) {
+    if (r >= 0 && i != r) continue;
+    c->hp[i].tile_cap = txh_cap;
+    c->hp[i].tile_tz = tz;
+  }
+  int rc = build_ideal_plan(c);
+  if (rc) {
+    c->hp = old;
+    build_ideal_plan(c);
+    return rc;
   }
   HIPCHECK(hipStreamSynchronize(c->stream));
   HIPCHECK(hipMemcpy(c->dP, c->hp.data(), c->nrep * sizeof(ReplicaParams),

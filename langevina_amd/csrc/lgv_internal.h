@@ -12,8 +12,9 @@ namespace lgv {
 // reference's expressions for the dt in use.
 struct ReplicaParams {
   void *fields[2];    // two buffers of 10 planes x 2 parities x half elements
-                      // (T = double/float); ReplicaState::cur selects the live
-                      // one, the ideal substep writes the other
+                      // (T = double/float); ReplicaState::cur_phi / cur_chg
+                      // select where the live phi / charge planes are, the
+                      // out-of-place kernels write the other buffer
   int N, NH;
   int64_t half;       // N^3 / 2
   double mass0, dmassdt, lambda, H, chi;
@@ -23,6 +24,7 @@ struct ReplicaParams {
   double dt_ideal;    // dt / scounts[1]
   double tau_ideal;   // dt_ideal / 2.0
   uint64_t base_key;
+  int tile_cap, tile_tz;  // ideal-kernel tile (see ideal_tile)
 };
 
 // Device-written state of one replica (clock, counters, per-step data).
@@ -30,19 +32,25 @@ struct ReplicaState {
   double t;
   uint64_t sweep_ctr;
   int order[5];      // charge sub-sweep order: directions[3], parities[2]
-  int cur;           // live field buffer (0/1)
+  int cur_phi;       // buffer (0/1) holding the live phi planes
+  int cur_chg;       // buffer (0/1) holding the live charge planes
+  int pad;
   double E_old, E_new;
 };
 
 // Tile of the fused ideal kernel: TXH half-columns x TY rows in x-y,
-// marching over TZ planes in z.  (TY+2)(2 TXH+2) <= kIdealLds.
-constexpr int kIdealLds = 680;
-__host__ __device__ inline void ideal_tile(int N, int &TXH, int &TY, int &TZ) {
+// marching over TZ planes in z.  txh_cap (8..64) and tz (4..64) are the
+// tunables (0 = defaults 32 and 16); (TY+2)(2 TXH+2) <= kIdealLds.
+constexpr int kIdealLds = 800;
+__host__ __device__ inline void ideal_tile(int N, int txh_cap, int tz, int &TXH, int &TY,
+                                           int &TZ) {
   const int NH = N / 2;
-  TXH = NH < 32 ? NH : 32;
+  const int cap = txh_cap > 0 ? txh_cap : 32;
+  const int zc = tz > 0 ? tz : 16;
+  TXH = NH < cap ? NH : cap;
   TY = 256 / TXH;
   if (TY > N) TY = N;
-  TZ = N < 16 ? N : 16;
+  TZ = N < zc ? N : zc;
 }
 
 // Per-block monitor partials (o4_stepper_monitor counters).
@@ -70,8 +78,8 @@ void launch_hb_phi(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaS
                    int ieo, int ctr_off, MonSlot *mon, hipStream_t st);
 void launch_shuffle(int nrep, ReplicaState *S, const ReplicaParams *P, int ctr_off,
                     hipStream_t st);
-void launch_charge(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
-                   int sub, int ctr_off, MonSlot *mon, hipStream_t st);
+void launch_charge_dir(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
+                       int nrep, int ixyz, int ctr_off, MonSlot *mon, hipStream_t st);
 void launch_ideal(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
                   int nrep, hipStream_t st);
 void launch_energy(bool fp32, const Plan &plan, const ReplicaParams *P,

@@ -15,12 +15,25 @@
 
 namespace lgv {
 
-// A select instead of p.fields[cur]: a runtime index into the register copy
-// of ReplicaParams would push the whole struct to scratch memory.
+// `cur` packs the live buffer of the phi planes (bit 0) and of the charge
+// planes (bit 1).  A select instead of p.fields[b]: a runtime index into the
+// register copy of ReplicaParams would push the whole struct to scratch.
 template <typename T>
 __device__ __forceinline__ T *plane(const ReplicaParams &p, int cur, int c, int par) {
-  T *base = reinterpret_cast<T *>(cur ? p.fields[1] : p.fields[0]);
+  const int b = c < 4 ? (cur & 1) : (cur >> 1);
+  T *base = reinterpret_cast<T *>(b ? p.fields[1] : p.fields[0]);
   return base + ((int64_t)(c * 2 + par)) * p.half;
+}
+__device__ __forceinline__ int live(const ReplicaState &s) {
+  return s.cur_phi | (s.cur_chg << 1);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Renumber blocks so that XCD k runs the k-th contiguous eighth of the
+// plan: blocks that share neighbour planes then share an L2.
+__device__ __forceinline__ int xcd_block() {
+  const int b = blockIdx.x, n = gridDim.x, q = n >> 3;
+  return b < (q << 3) ? (b & 7) * q + (b >> 3) : b;
 }
 
 __device__ __forceinline__ int wrapp(int a, int N) { return a == N ? 0 : a; }
@@ -69,9 +82,10 @@ __global__ __launch_bounds__(kBlock) void k_hb_phi(const ReplicaParams *__restri
                                                    const int *__restrict__ blk_rep,
                                                    const int *__restrict__ blk_first, int ieo,
                                                    int ctr_off, MonSlot *__restrict__ mon) {
-  const int r = blk_rep[blockIdx.x];
+  const int bid = xcd_block();
+  const int r = blk_rep[bid];
   const ReplicaParams p = P[r];
-  const int64_t idx = (int64_t)(blockIdx.x - blk_first[r]) * kBlock + threadIdx.x;
+  const int64_t idx = (int64_t)(bid - blk_first[r]) * kBlock + threadIdx.x;
   long long down = 0, upyes = 0, upno = 0;
   double dSd = 0., dSu = 0.;
   if (idx < p.half) {
@@ -99,11 +113,11 @@ __global__ __launch_bounds__(kBlock) void k_hb_phi(const ReplicaParams *__restri
     double phi_o[4], phi_n[4];
 #pragma unroll
     for (int l = 0; l < 4; l++) {
-      const T *fq = plane<T>(p, s.cur, l, q);
+      const T *fq = plane<T>(p, live(s), l, q);
       const double heff = ((double)fq[nb[0]] + (double)fq[nb[1]]) +
                           ((double)fq[nb[2]] + (double)fq[nb[3]]) +
                           ((double)fq[nb[4]] + (double)fq[nb[5]]);
-      phi_o[l] = (double)plane<T>(p, s.cur, l, ieo)[idx];
+      phi_o[l] = (double)plane<T>(p, live(s), l, ieo)[idx];
       phi_n[l] = phi_o[l] + rdtg * variance1(g);
       s_o += phi_o[l] * phi_o[l];
       s_n += phi_n[l] * phi_n[l];
@@ -126,10 +140,10 @@ __global__ __launch_bounds__(kBlock) void k_hb_phi(const ReplicaParams *__restri
     }
     if (accept) {
 #pragma unroll
-      for (int l = 0; l < 4; l++) plane<T>(p, s.cur, l, ieo)[idx] = (T)phi_n[l];
+      for (int l = 0; l < 4; l++) plane<T>(p, live(s), l, ieo)[idx] = (T)phi_n[l];
     }
   }
-  block_monitor_add(mon + blockIdx.x, down, upyes, upno, dSd, dSu);
+  block_monitor_add(mon + bid, down, upyes, upno, dSd, dSu);
 }
 
 // ------------------------------------------------------------------------
@@ -145,64 +159,107 @@ __global__ void k_shuffle(ReplicaState *S, const ReplicaParams *P, int nrep, int
 }
 
 // ------------------------------------------------------------------------
-// One (direction, parity) sub-sweep of ModelGChargeHB::step
-// (Stepper.cxx:474-518 with modelg_update_charge_pair :419-441).  Pairs
-// (A, A+e_dir) are disjoint, so each thread owns its pair.
+// The two sub-sweeps of one direction of ModelGChargeHB::step (the shuffled
+// order always runs both parities of a direction back to back,
+// Stepper.cxx:471-473) in one pass.  Thread = site A' of the second parity
+// e2; along the direction its line reads L = A'-e, A', B' = A'+e, R = A'+2e.
+// First sub-sweep (parity e1): pairs (L, A') and (B', R), keyed by L and B';
+// second: pair (A', B') keyed by A' on the first sub-sweep's results.  Each
+// first-parity pair is computed by the two threads that need it (identical
+// counter streams, identical bits); the monitor counts it once, from the
+// thread on its right.  Reads the live charge buffer, writes the other one.
+__device__ __forceinline__ void charge_pair(double nA, double nB, double rms, double chi,
+                                            Xoro &g, double &outA, double &outB,
+                                            long long &down, long long &upyes,
+                                            long long &upno, double &dSd, double &dSu,
+                                            bool count) {
+  double q = rms * variance1(g);
+  const double dS = ((nA - q) * (nA - q) + (nB + q) * (nB + q) - nA * nA - nB * nB) /
+                    (2. * chi);
+  if (dS < 0) {
+    if (count) { down++; dSd += dS; }
+  } else {
+    const double rr = uniform01(g);
+    if (rr < exp(-dS)) {
+      if (count) { upyes++; dSu += dS; }
+    } else {
+      if (count) upno++;
+      q = 0.;
+    }
+  }
+  outA = nA + (-q);
+  outB = nB + q;
+}
+
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_charge(const ReplicaParams *__restrict__ P,
-                                                   const ReplicaState *__restrict__ S,
-                                                   const int *__restrict__ blk_rep,
-                                                   const int *__restrict__ blk_first, int sub,
-                                                   int ctr_off, MonSlot *__restrict__ mon) {
-  const int r = blk_rep[blockIdx.x];
+__global__ __launch_bounds__(kBlock) void k_charge_dir(const ReplicaParams *__restrict__ P,
+                                                       const ReplicaState *__restrict__ S,
+                                                       const int *__restrict__ blk_rep,
+                                                       const int *__restrict__ blk_first,
+                                                       int ixyz, int ctr_off,
+                                                       MonSlot *__restrict__ mon) {
+  const int bid = xcd_block();
+  const int r = blk_rep[bid];
   const ReplicaParams p = P[r];
-  const int64_t idx = (int64_t)(blockIdx.x - blk_first[r]) * kBlock + threadIdx.x;
+  const int64_t idx = (int64_t)(bid - blk_first[r]) * kBlock + threadIdx.x;
   long long down = 0, upyes = 0, upno = 0;
   double dSd = 0., dSu = 0.;
   if (idx < p.half) {
     const ReplicaState s = S[r];
-    // indexed loads from global memory: a runtime index into the private
-    // copy would make the compiler stage the struct in LDS
-    const int dir = S[r].order[sub >> 1];
-    const int eoA = S[r].order[3 + (sub & 1)];
+    const int cur = live(s), nxt = cur ^ 2;
+    const int dir = S[r].order[ixyz];
+    const int e1 = S[r].order[3], e2 = S[r].order[4];
     const int N = p.N, NH = p.NH;
     const int xh = (int)(idx % NH);
     const int64_t rowi = idx / NH;
     const int y = (int)(rowi % N), z = (int)(rowi / N);
-    const int x = site_x(xh, y, z, eoA);
-    int64_t idxB;
-    if (dir == 0) idxB = rowi * NH + (wrapp(x + 1, N) >> 1);
-    else if (dir == 1) idxB = ((int64_t)z * N + wrapp(y + 1, N)) * NH + xh;
-    else idxB = ((int64_t)wrapp(z + 1, N) * N + y) * NH + xh;
+    const int x = site_x(xh, y, z, e2);
+    // the four sites along the direction: L, A', B', R (parities e1 e2 e1 e2)
+    const int a0 = dir == 0 ? x : (dir == 1 ? y : z);
+    const int am = a0 == 0 ? N - 1 : a0 - 1;
+    const int ap = a0 + 1 == N ? 0 : a0 + 1;
+    const int app = ap + 1 == N ? 0 : ap + 1;
+    int64_t ii[4];
+    uint64_t lin[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int a = k == 0 ? am : (k == 1 ? a0 : (k == 2 ? ap : app));
+      const int xx = dir == 0 ? a : x, yy = dir == 1 ? a : y, zz = dir == 2 ? a : z;
+      ii[k] = ((int64_t)zz * N + yy) * NH + (xx >> 1);
+      lin[k] = ((uint64_t)zz * N + yy) * N + xx;
+    }
     const double chi = p.chi, rms = p.rms;
-    Xoro g = xoro_seeded(site_key(stream_key(p.base_key, s.sweep_ctr + ctr_off),
-                                  ((uint64_t)z * N + y) * N + x));
+    const uint64_t k1 = stream_key(p.base_key, s.sweep_ctr + ctr_off);
+    const uint64_t k2 = stream_key(p.base_key, s.sweep_ctr + ctr_off + 1);
+    Xoro gL = xoro_seeded(site_key(k1, lin[0]));
+    Xoro gB = xoro_seeded(site_key(k1, lin[2]));
+    Xoro gA = xoro_seeded(site_key(k2, lin[1]));
+    long long d0 = 0, y0 = 0, n0 = 0;
+    double sd0 = 0., su0 = 0.;
 #pragma unroll
     for (int L = 4; L < 10; L++) {
-      T *pa = plane<T>(p, s.cur, L, eoA);
-      T *pb = plane<T>(p, s.cur, L, eoA ^ 1);
-      const double nA = (double)pa[idx], nB = (double)pb[idxB];
-      double q = rms * variance1(g);
-      const double dS = ((nA - q) * (nA - q) + (nB + q) * (nB + q) - nA * nA - nB * nB) /
-                        (2. * chi);
-      if (dS < 0) {
-        down++;
-        dSd += dS;
-      } else {
-        const double rr = uniform01(g);
-        if (rr < exp(-dS)) {
-          upyes++;
-          dSu += dS;
-        } else {
-          upno++;
-          q = 0.;
-        }
-      }
-      pa[idx] = (T)(nA + (-q));
-      pb[idxB] = (T)(nB + q);
+      const double vL = (double)plane<T>(p, cur, L, e1)[ii[0]];
+      const double vA = (double)plane<T>(p, cur, L, e2)[ii[1]];
+      const double vB = (double)plane<T>(p, cur, L, e1)[ii[2]];
+      const double vR = (double)plane<T>(p, cur, L, e2)[ii[3]];
+      double tL, a1, b1, tR;
+      charge_pair(vL, vA, rms, chi, gL, tL, a1, down, upyes, upno, dSd, dSu, true);
+      charge_pair(vB, vR, rms, chi, gB, b1, tR, d0, y0, n0, sd0, su0, false);
+      // the first sub-sweep's results as stored (rounded in FP32 storage)
+      a1 = (double)(T)a1;
+      b1 = (double)(T)b1;
+      double a2, b2;
+      charge_pair(a1, b1, rms, chi, gA, a2, b2, down, upyes, upno, dSd, dSu, true);
+      plane<T>(p, nxt, L, e2)[ii[1]] = (T)a2;
+      plane<T>(p, nxt, L, e1)[ii[2]] = (T)b2;
     }
   }
-  block_monitor_add(mon + blockIdx.x, down, upyes, upno, dSd, dSu);
+  block_monitor_add(mon + bid, down, upyes, upno, dSd, dSu);
+}
+
+__global__ void k_flip_chg(ReplicaState *S, int nrep) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < nrep) S[r].cur_chg ^= 1;
 }
 
 // ------------------------------------------------------------------------
@@ -228,14 +285,16 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
                                                         const int *__restrict__ blk_rep,
                                                         const int *__restrict__ blk_first) {
   __shared__ double lds[4][kIdealLds];
-  const int r = blk_rep[blockIdx.x];
-  const ReplicaParams p = P[r];
-  const int cur = S[r].cur, nxt = cur ^ 1;
+  const int bid = xcd_block();
+  const int r = blk_rep[bid];
+  // a reference, not a copy: a private copy would be staged in scratch memory
+  const ReplicaParams &p = P[r];
+  const int cur = live(S[r]), nxt = cur ^ 3;
   const int N = p.N, NH = p.NH;
   int TXH, TY, TZ;
-  ideal_tile(N, TXH, TY, TZ);
+  ideal_tile(N, p.tile_cap, p.tile_tz, TXH, TY, TZ);
   const int ntx = (NH + TXH - 1) / TXH, nty = (N + TY - 1) / TY;
-  int b = blockIdx.x - blk_first[r];
+  int b = bid - blk_first[r];
   const int bx = b % ntx;
   b /= ntx;
   const int by = b % nty, bz = b / nty;
@@ -260,7 +319,7 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
 
   double prv[2][4], cen[2][4], nex[2][4];  // rotated phi, own sites, z-1, z, z+1
   double chg[2][6], chn[2][6];             // raw charges of planes z and z+1
-  auto rotate_own = [&](int z, double out[2][4], double ch[2][6]) {
+  auto rotate_own = [&](int z, double out[2][4], double ch[2][6]) __attribute__((always_inline)) {
     if (!own) return;
     const int64_t i = ((int64_t)z * N + y) * NH + xh;
 #pragma unroll
@@ -276,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
       for (int c = 0; c < 6; c++) ch[s][c] = f[4 + c];
     }
   };
-  auto fill_lds = [&](int z, const double o[2][4]) {
+  auto fill_lds = [&](int z, const double o[2][4]) __attribute__((always_inline)) {
     if (own) {
 #pragma unroll
       for (int s = 0; s < 2; s++)
@@ -370,7 +429,10 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
 
 __global__ void k_flip(ReplicaState *S, int nrep) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < nrep) S[r].cur ^= 1;
+  if (r < nrep) {
+    S[r].cur_phi ^= 1;
+    S[r].cur_chg ^= 1;
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -382,10 +444,11 @@ __global__ __launch_bounds__(kBlock) void k_energy(const ReplicaParams *__restri
                                                    const int *__restrict__ blk_rep,
                                                    const int *__restrict__ blk_first,
                                                    EnergySlot *__restrict__ slots) {
-  const int r = blk_rep[blockIdx.x];
+  const int bid = xcd_block();
+  const int r = blk_rep[bid];
   const ReplicaParams p = P[r];
-  const int cur = S[r].cur;
-  int64_t idx = (int64_t)(blockIdx.x - blk_first[r]) * kBlock + threadIdx.x;
+  const int cur = live(S[r]);
+  int64_t idx = (int64_t)(bid - blk_first[r]) * kBlock + threadIdx.x;
   double g2 = 0., hE = 0., n2 = 0.;
   if (idx < 2 * p.half) {
     const int par = idx >= p.half;
@@ -427,7 +490,7 @@ __global__ __launch_bounds__(kBlock) void k_energy(const ReplicaParams *__restri
     for (int i = 1; i < kBlock / 64; i++) {
       ksum_merge(a, sh[0][i]); ksum_merge(b, sh[1][i]); ksum_merge(c, sh[2][i]);
     }
-    slots[blockIdx.x] = EnergySlot{a.s, a.c, b.s, b.c, c.s, c.c};
+    slots[bid] = EnergySlot{a.s, a.c, b.s, b.c, c.s, c.c};
   }
 }
 
@@ -499,7 +562,7 @@ template <typename T>
 __global__ void k_aos_to_rb(const ReplicaParams *P, const ReplicaState *S, int replica,
                             const double *aos) {
   const ReplicaParams p = P[replica];
-  const int cur = S[replica].cur;
+  const int cur = live(S[replica]);
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t V = 2 * p.half;
   if (s >= V) return;
@@ -516,7 +579,7 @@ template <typename T>
 __global__ void k_rb_to_aos(const ReplicaParams *P, const ReplicaState *S, int replica,
                             double *aos) {
   const ReplicaParams p = P[replica];
-  const int cur = S[replica].cur;
+  const int cur = live(S[replica]);
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t V = 2 * p.half;
   if (s >= V) return;
@@ -577,7 +640,7 @@ __global__ __launch_bounds__(kBlock) void k_measure_cols(const ReplicaParams *P,
                                                          const ReplicaState *S, int replica,
                                                          int along_z, double *part) {
   const ReplicaParams p = P[replica];
-  const int cur = S[replica].cur;
+  const int cur = live(S[replica]);
   const int N = p.N;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)N * N) return;
@@ -671,14 +734,15 @@ void launch_shuffle(int nrep, ReplicaState *S, const ReplicaParams *P, int ctr_o
   k_shuffle<<<nblk(nrep, 64), 64, 0, st>>>(S, P, nrep, ctr_off);
 }
 
-void launch_charge(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
-                   int sub, int ctr_off, MonSlot *mon, hipStream_t st) {
+void launch_charge_dir(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
+                       int nrep, int ixyz, int ctr_off, MonSlot *mon, hipStream_t st) {
   if (fp32)
-    k_charge<float><<<plan.nblocks, kBlock, 0, st>>>(P, S, plan.d_blk_rep, plan.d_blk_first,
-                                                     sub, ctr_off, mon);
+    k_charge_dir<float><<<plan.nblocks, kBlock, 0, st>>>(P, S, plan.d_blk_rep,
+                                                         plan.d_blk_first, ixyz, ctr_off, mon);
   else
-    k_charge<double><<<plan.nblocks, kBlock, 0, st>>>(P, S, plan.d_blk_rep, plan.d_blk_first,
-                                                      sub, ctr_off, mon);
+    k_charge_dir<double><<<plan.nblocks, kBlock, 0, st>>>(P, S, plan.d_blk_rep,
+                                                          plan.d_blk_first, ixyz, ctr_off, mon);
+  k_flip_chg<<<nblk(nrep, 64), 64, 0, st>>>(S, nrep);
 }
 
 void launch_ideal(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,

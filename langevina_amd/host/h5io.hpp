@@ -18,6 +18,10 @@
 
 namespace lgv_host {
 
+// H5F_ACC_* expand to calls of H5check()/H5open(), which are reached here
+// only through dlsym; these are their values (HDF5 1.10 H5Fpublic.h:50-52).
+constexpr unsigned kAccRdonly = 0x0000u, kAccRdwr = 0x0001u, kAccTrunc = 0x0002u;
+
 struct H5Api {
   void *lib = nullptr;
   decltype(&H5open) open_;
@@ -174,10 +178,10 @@ class H5File {
       FILE *f = fopen(name.c_str(), "rb");
       if (f) {
         fclose(f);
-        id = h.Fopen(name.c_str(), H5F_ACC_RDWR, H5P_DEFAULT);
+        id = h.Fopen(name.c_str(), kAccRdwr, H5P_DEFAULT);
       }
     }
-    if (id < 0) id = h.Fcreate(name.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+    if (id < 0) id = h.Fcreate(name.c_str(), kAccTrunc, H5P_DEFAULT, H5P_DEFAULT);
     if (id < 0) throw std::runtime_error("cannot open " + name);
   }
   ~H5File() {
@@ -205,7 +209,7 @@ inline void read_o4fields(const std::string &fname, int N, double *u) {
   if (!probe)
     throw std::string("Unable to open file in restart mode with filename = ") + fname;
   fclose(probe);
-  hid_t fid = h.Fopen(fname.c_str(), H5F_ACC_RDONLY, H5P_DEFAULT);
+  hid_t fid = h.Fopen(fname.c_str(), kAccRdonly, H5P_DEFAULT);
   if (fid < 0) throw std::string("Unable to open file in restart mode with filename = ") + fname;
   hid_t ds = h.Dopen2(fid, "o4fields", H5P_DEFAULT);
   if (ds < 0) {

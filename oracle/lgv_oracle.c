@@ -217,6 +217,30 @@ uint64_t orc_site_key(uint64_t skey, uint64_t site) {
   return orc_mix64(skey ^ orc_mix64(site + 0x3c6ef372fe94f82bULL));
 }
 
+/* First moments of the keyed streams: mean[k] of the k-th canonical output
+ * and cross[k] = E[u_k u_{k+1}] over the streams of sites 0..n-1 of one
+ * counter (a statistical check of the per-site scheme). */
+void orc_stream_moments(uint64_t base, uint64_t ctr, long n, int nout, double *mean,
+                        double *cross) {
+  const uint64_t skey = orc_stream_key(base, ctr);
+  for (int k = 0; k < nout; k++) mean[k] = cross[k] = 0.;
+  for (long i = 0; i < n; i++) {
+    orc_xoro g;
+    orc_xoro_seed(&g, orc_site_key(skey, (uint64_t)i));
+    double prev = 0.;
+    for (int k = 0; k < nout; k++) {
+      const double u = orc_canonical(orc_xoro_next(&g));
+      mean[k] += u;
+      if (k > 0) cross[k - 1] += prev * u;
+      prev = u;
+    }
+  }
+  for (int k = 0; k < nout; k++) {
+    mean[k] /= (double)n;
+    cross[k] /= (double)n;
+  }
+}
+
 /* ======================================================================
  * O(4) algebra, ModelG/O4AlgebraHelper.h
  * ==================================================================== */

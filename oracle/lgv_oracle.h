@@ -52,6 +52,8 @@ uint64_t orc_mix64(uint64_t z);
 uint64_t orc_base_key(uint32_t seed32);
 uint64_t orc_stream_key(uint64_t base, uint64_t ctr);
 uint64_t orc_site_key(uint64_t skey, uint64_t site);
+void orc_stream_moments(uint64_t base, uint64_t ctr, long n, int nout, double *mean,
+                        double *cross);
 
 /* ---------------- O(4) algebra ---------------- */
 void orc_expmul(const double *v, double *w);

@@ -96,6 +96,7 @@ def lib():
         L.orc_stream_key.restype = C.c_uint64
         L.orc_site_key.argtypes = [C.c_uint64, C.c_uint64]
         L.orc_site_key.restype = C.c_uint64
+        L.orc_stream_moments.argtypes = [C.c_uint64, C.c_uint64, C.c_long, C.c_int, dp, dp]
         L.orc_expmul.argtypes = [dp, dp]
         L.orc_quatmul.argtypes = [dp, dp, dp]
         L.orc_o4_rotation.argtypes = [dp, dp, dp]

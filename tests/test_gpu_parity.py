@@ -206,3 +206,22 @@ def test_64_cubed_step(L, orc):
     lat.step(0.04)
     assert rel_err(ctx.get_field(), lat.u) <= 1e-12
     ctx.close()
+
+
+@pytest.mark.parametrize("N", [16, 48])
+def test_ideal_tile_does_not_change_results(L, orc, N):
+    log("tiles", N)
+    from langevina_amd.autotune import set_ideal_tile
+    u = np.random.default_rng(9).normal(size=(N, N, N, 10)) * 0.5
+    ref = None
+    for cap, tz in ((32, 16), (8, 4), (16, 32), (64, 8)):
+        ctx = L.Context(L.model_params(N, seed=3, **PAR))
+        set_ideal_tile(ctx, 0, cap, tz)
+        ctx.set_field(u)
+        ctx.step(0.04)
+        got = ctx.get_field()
+        ctx.close()
+        if ref is None:
+            ref = got
+        else:
+            np.testing.assert_array_equal(got, ref)

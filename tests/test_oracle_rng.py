@@ -147,3 +147,17 @@ def test_site_keys_distinct_and_mixed(orc):
     k = orc.site_key(sk, 17)
     e = orc.Engine(seed=k)
     assert e.next() == ((0x01d353e5f3993bb0 + k) + 0x7b9c0df6cb193b20 * (k + 1)) % 2**64
+
+
+def test_per_site_streams_are_unbiased(orc):
+    # first six outputs of 2e6 keyed streams (three counters): means 1/2 and
+    # lag-one cross moments 1/4 within 5 standard errors
+    import numpy as np
+    n, nout = 2_000_000, 6
+    base = orc.base_key(orc.seed_seq_generate([12345], 1)[0])
+    for ctr in (0, 1, 2**62 + 7):
+        mean, cross = np.zeros(nout), np.zeros(nout)
+        orc.lib().orc_stream_moments(base, ctr, n, nout, mean, cross)
+        assert np.all(np.abs(mean - 0.5) < 5 * (1 / 12 / n) ** 0.5), mean
+        sd_cross = ((1 / 9) - (1 / 16)) ** 0.5 / n ** 0.5
+        assert np.all(np.abs(cross[:nout - 1] - 0.25) < 5 * sd_cross), cross
