@@ -2,7 +2,7 @@
 """Throughput of the ModelG time step (PV2HBSplit23) on one MI355X.
 
     python bench.py [--steps K] [--warmup W] [--size 256] [--precision fp64|fp32]
-                    [--replicas batched] [--no-cpu-baseline]
+                    [--graph] [--batched] [--drift] [--no-cpu-baseline] [--calibrate-copy]
 
 One "step" is PV2HBSplit::step(dt) (3 ideal substeps, 12 phi Metropolis
 half-sweeps, 6 charge sub-sweeps) applied to the whole lattice; the metric
@@ -19,11 +19,46 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
-# algorithmic bytes per site per launch, FP64 fields (halve for FP32):
-# ideal substep read+write 80+80, phi half-sweep read 32 + write 16,
-# charge sub-sweep read 48 + write 48
+# Algorithmic bytes per site per launch, FP64 fields (halve for FP32):
+# ideal substep (k_ideal_fused) read 80 + write 80; phi half-sweep
+# (k_hb_phi) read all phi 32 + write one parity 16; charge direction
+# (k_charge_dir, both sub-sweeps of one direction) read 48 + write 48.
 ALG_BYTES = {"ideal": 160, "phi": 48, "charge": 96}
-ALG_BYTES_PER_SITE_UPDATE = 3 * 160 + 12 * 48 + 6 * 96  # 1632
+KERNEL = {"ideal": "k_ideal_fused", "phi": "k_hb_phi", "charge": "k_charge_dir"}
+# Per site-update with one pass per dependency level of the reference
+# (six separate charge sub-sweeps): 3*160 + 12*48 + 6*96.
+ALG_BYTES_PER_SITE_UPDATE = 1632
+# The same for this schedule, whose charge launches cover two sub-sweeps.
+SCHEDULE_BYTES_PER_SITE_UPDATE = 3 * 160 + 12 * 48 + 3 * 96  # 1344
+# HBM bytes per launch per site measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (kernel, precision, N) -> (fetch, write), each scaled by the
+# copy-kernel calibration (bench.py --calibrate-copy); the summary of those
+# runs is profiles/modelg_256_hbm_traffic.md.
+MEASURED_TRAFFIC = {}
+
+
+
+def traffic(cls, precision, N):
+    """Measured HBM bytes per launch of the class's kernel at N^3, or None."""
+    m = MEASURED_TRAFFIC.get((KERNEL[cls], precision, N))
+    return None if m is None else (m[0] + m[1]) * N ** 3
+
+
+def calibrate_copy(mib=1024, reps=5):
+    """A plain device copy of known size (torch.neg over mib MiB of FP64,
+    out of place), run reps times: under rocprofv3 --pmc its FETCH_SIZE and
+    WRITE_SIZE per dispatch, against mib MiB read and written, give the
+    counters' scale factor."""
+    import torch
+    n = mib * 2 ** 20 // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_read": n * 8, "bytes_written": n * 8,
+                      "reps": reps, "kernel": "torch.neg (elementwise, FP64)"}), flush=True)
+
 
 PARAMS = dict(mass0=-4.813, lam=1.0, H=0.01, chi=2.0, gamma=1.0)
 DT = 0.04
@@ -132,11 +167,53 @@ def bench_batched(args):
         "one_at_a_time": {"value": seq_value, "ms_per_step": seq_el * 1e3 / args.steps},
         "batched_over_one_at_a_time": value / seq_value,
         "step_roofline_frac": value * ALG_BYTES_PER_SITE_UPDATE * esz / 8 / HBM_PEAK,
-        "roofline": {"bound": "hbm", "kernel": {"ideal": "k_ideal_fused", "phi": "k_hb_phi",
-                                                 "charge": "k_charge"}[dom],
+        "roofline": {"bound": "hbm", "kernel": KERNEL[dom],
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None},
     }
+    print(json.dumps(out), flush=True)
+
+
+def precision_drift(args, steps=500, every=50):
+    """FP32 fields against FP64 on the same lattice, seed and noise streams:
+    500 steps, a measurement every 50.  Per measurement: <|phi|> = |phi_bar|,
+    |phi_bar|^2 and the k = 1 wall correlator G1 = N^3/3 sum_d sum_a
+    |phi~_a(k=1)|^2 (phi~ from the device r2c of the z/y/x walls, k in units
+    of 2 pi / N).  The susceptibility is N^3 (<|phi_bar|^2> - <|phi_bar|>^2)
+    over the measurement series."""
+    import numpy as np
+    import langevina_amd as L
+    N = args.size
+    ctxs = {}
+    for name, prec in (("fp64", L.FP64), ("fp32", L.FP32)):
+        c = L.Context(L.model_params(N, seed=42, **PARAMS), precision=prec)
+        c.init_gaussian_charges()
+        c.use_graph(True)
+        ctxs[name] = c
+    series = {k: {"absphi": [], "phi2": [], "G1": []} for k in ctxs}
+    for _ in range(steps // every):
+        for name, c in ctxs.items():
+            c.run(DT, every)
+            m = c.measure_full()
+            pb = m["oavg"][:4]
+            wk = m["wall_k"][:, :4, 1]
+            series[name]["absphi"].append(float(np.sqrt(np.sum(pb * pb))))
+            series[name]["phi2"].append(float(np.sum(pb * pb)))
+            series[name]["G1"].append(float(N ** 3 / 3.0 * np.sum(np.abs(wk) ** 2)))
+    for c in ctxs.values():
+        c.close()
+    out = {"drift": "FP32 fields vs FP64", "N": N, "steps": steps, "measure_every": every,
+           "dt": DT, "params": PARAMS, "seed": 42}
+    for obs in ("absphi", "G1"):
+        a = np.array(series["fp64"][obs])
+        b = np.array(series["fp32"][obs])
+        rel = np.abs(b - a) / np.abs(a)
+        out[obs] = {"fp64": a.tolist(), "fp32": b.tolist(), "max_rel_dev": float(rel.max()),
+                    "final_rel_dev": float(rel[-1])}
+    chi = {k: float(N ** 3 * (np.mean(v["phi2"]) - np.mean(v["absphi"]) ** 2))
+           for k, v in series.items()}
+    out["susceptibility"] = {"fp64": chi["fp64"], "fp32": chi["fp32"],
+                             "rel_dev": abs(chi["fp32"] - chi["fp64"]) / abs(chi["fp64"])}
     print(json.dumps(out), flush=True)
 
 
@@ -152,11 +229,21 @@ def main():
                     help="96 mixed-size replicas in one context (use with --precision fp32)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run the copy kernel used to scale HBM counters")
+    ap.add_argument("--drift", action="store_true",
+                    help="report FP32-field observables against FP64 (500 steps)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the lattice and every replica "
               "live on a single MI355X); --gpus must be 1", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
+    if args.drift:
+        precision_drift(args)
+        return
     if args.batched:
         bench_batched(args)
         return
@@ -217,10 +304,10 @@ def main():
                    "N": N, "dt": DT, "params": PARAMS, "parallelism": "1 GPU",
                    "graph": bool(args.graph)},
         "step_roofline_frac": value * ALG_BYTES_PER_SITE_UPDATE * esz / 8 / HBM_PEAK,
-        "roofline": {"bound": "hbm", "kernel": {"ideal": "k_ideal_rot1+k_ideal_kick_rot2",
-                                                 "phi": "k_hb_phi", "charge": "k_charge"}[dom],
+        "step_schedule_bytes_frac": value * SCHEDULE_BYTES_PER_SITE_UPDATE * esz / 8 / HBM_PEAK,
+        "roofline": {"bound": "hbm", "kernel": KERNEL[dom],
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "frac": achieved / HBM_PEAK, "traffic": traffic(dom, args.precision, N),
                      "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": per_launch_s * 1e6},
         "kernels": kernels,
     }

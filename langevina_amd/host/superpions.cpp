@@ -254,7 +254,14 @@ int main(int argc, char **argv) {
     return 1;
   }
   printf("Current version: langevina_amd lgv %d\n", lgv_version());
-  ModelAData data(inputs);
+  std::unique_ptr<ModelAData> pdata;
+  try {
+    pdata.reset(new ModelAData(inputs));
+  } catch (const std::exception &e) {
+    fprintf(stderr, "%s\n", e.what());
+    return 1;
+  }
+  ModelAData &data = *pdata;
   if (quit) return 0;
 
   const std::string &etype = data.ahandler.evolverType;

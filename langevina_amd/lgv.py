@@ -57,8 +57,19 @@ EXPORTS = [
     "lgv_use_graph", "lgv_ideal_step", "lgv_hb_phi_step", "lgv_charge_step",
     "lgv_init_gaussian_charges", "lgv_thermalize", "lgv_monitor_get", "lgv_energy_get",
     "lgv_rng_get_state", "lgv_rng_set_state", "lgv_measure", "lgv_synchronize",
-    "lgv_field_device_ptr", "lgv_timing_enable", "lgv_timing_get",
+    "lgv_field_device_ptr", "lgv_timing_enable", "lgv_timing_get", "lgv_measure_full",
+    "lgv_set_mass", "lgv_monitor_reset", "lgv_set_ideal_tile",
+    "lgv_em_last_error", "lgv_em_create", "lgv_em_set_stream", "lgv_em_forward",
+    "lgv_em_backward", "lgv_em_destroy",
 ]
+
+
+class Measurement(C.Structure):
+    """lgv_measurement: output pointers of lgv_measure_full (NULL = skip)."""
+    _fields_ = [("wall", C.POINTER(C.c_double)), ("wallphase", C.POINTER(C.c_double)),
+                ("oavg", C.POINTER(C.c_double)), ("wall_k", C.POINTER(C.c_double)),
+                ("wall_k_rotated", C.POINTER(C.c_double)),
+                ("wallphase_k", C.POINTER(C.c_double))]
 
 _lib = None
 
@@ -95,6 +106,10 @@ def lib():
     L.lgv_timing_enable.argtypes = [ctx, C.c_int]
     L.lgv_timing_get.argtypes = [ctx, dp, np.ctypeslib.ndpointer(dtype=np.int64,
                                                                  flags="C_CONTIGUOUS")]
+    L.lgv_measure_full.argtypes = [ctx, C.c_int, C.POINTER(Measurement)]
+    L.lgv_set_mass.argtypes = [ctx, C.c_int, C.c_double, C.c_double]
+    L.lgv_monitor_reset.argtypes = [ctx]
+    L.lgv_set_ideal_tile.argtypes = [ctx, C.c_int, C.c_int, C.c_int]
     L.lgv_synchronize.argtypes = [ctx]
     L.lgv_field_device_ptr.argtypes = [ctx, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_int)]
     _lib = L
@@ -228,6 +243,26 @@ class Context:
         oavg = np.zeros(11)
         _check(lib().lgv_measure(self.h, replica, wall.reshape(-1), wallphase.reshape(-1), oavg))
         return wall, wallphase, oavg
+
+    def measure_full(self, replica=0):
+        """Measurer::measure: walls, OAverage and the k-space walls
+        (wall_k, wall_k_rotated, wallphase_k as complex arrays, k = 0..N/2)."""
+        N = self.N[replica]
+        K = N // 2 + 1
+        out = {"wall": np.zeros((3, 11, N)), "wallphase": np.zeros((3, 14, N)),
+               "oavg": np.zeros(11), "wall_k": np.zeros((3, 11, K, 2)),
+               "wall_k_rotated": np.zeros((3, 14, K, 2)), "wallphase_k": np.zeros((3, 14, K, 2))}
+        m = Measurement(*[v.ctypes.data_as(C.POINTER(C.c_double)) for v in out.values()])
+        _check(lib().lgv_measure_full(self.h, replica, C.byref(m)))
+        for k in ("wall_k", "wall_k_rotated", "wallphase_k"):
+            out[k] = out[k][..., 0] + 1j * out[k][..., 1]
+        return out
+
+    def set_mass(self, mass0, dmassdt=0.0, replica=-1):
+        _check(lib().lgv_set_mass(self.h, replica, mass0, dmassdt))
+
+    def monitor_reset(self):
+        _check(lib().lgv_monitor_reset(self.h))
 
     def synchronize(self):
         _check(lib().lgv_synchronize(self.h))

@@ -1,0 +1,97 @@
+"""The C-ABI library (include/lgv.h) on a machine without a GPU: it loads,
+exports every entry point the header declares, and rejects bad arguments
+with the documented codes before touching a device."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "lgv.h")
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^[A-Za-z_][\w \*]*?\b(lgv_\w+)\s*\(", text, flags=re.M)))
+
+
+@pytest.fixture(scope="module")
+def L():
+    from langevina_amd import lgv
+    if not os.path.exists(lgv.LIB_PATH):
+        pytest.fail(f"{lgv.LIB_PATH} missing: build it with python -m langevina_amd.build")
+    return lgv
+
+
+def test_header_declares_the_interface():
+    names = declared()
+    for must in ("lgv_create", "lgv_step", "lgv_run", "lgv_measure_full", "lgv_em_forward",
+                 "lgv_em_backward", "lgv_rng_get_state"):
+        assert must in names
+    assert len(names) >= 35
+
+
+def test_library_exports_every_declared_symbol(L):
+    lib = L.lib()
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert sorted(L.EXPORTS) == declared()
+
+
+def test_version_and_error_text(L):
+    lib = L.lib()
+    assert lib.lgv_version() >= 1
+    assert isinstance(lib.lgv_last_error(), bytes)
+
+
+def _create(L, params, **kw):
+    from langevina_amd.lgv import Config, ModelParams
+    arr = (ModelParams * len(params))(*params)
+    cfg = Config()
+    cfg.nreplicas = len(params)
+    cfg.replicas = C.cast(arr, C.POINTER(ModelParams))
+    cfg.precision = kw.get("precision", 0)
+    cfg.scounts[0], cfg.scounts[1] = kw.get("scounts", (2, 3))
+    h = C.c_void_p()
+    rc = L.lib().lgv_create(C.byref(cfg), C.byref(h))
+    return rc, L.lib().lgv_last_error().decode()
+
+
+@pytest.mark.parametrize("bad, text", [
+    (dict(NX=15), "even"),
+    (dict(NX=2), "even"),
+    (dict(chi=0.0), "chi"),
+])
+def test_create_rejects_bad_replicas(L, bad, text):
+    p = L.model_params(16)
+    for k, v in bad.items():
+        setattr(p, k, v)
+    rc, msg = _create(L, [p])
+    assert rc == -1 and text in msg
+
+
+def test_create_rejects_bad_config(L):
+    rc, msg = _create(L, [L.model_params(16)], precision=7)
+    assert rc == -1 and "precision" in msg
+    rc, msg = _create(L, [L.model_params(16)], scounts=(2, 100))
+    assert rc == -1 and "scounts" in msg
+    lib = L.lib()
+    assert lib.lgv_create(None, None) == -1
+    assert lib.lgv_step(None, C.c_double(0.04)) == -1
+
+
+def test_em_create_rejects_bad_arguments(L):
+    from langevina_amd.em import EmParams, _bind
+    lib = _bind()
+    p = EmParams()
+    p.N, p.dt = 2, 0.02
+    h = C.c_void_p()
+    assert lib.lgv_em_create(C.byref(p), 10, 5, 0, C.byref(h)) == -1
+    assert b"N=2" in lib.lgv_em_last_error()
+    p.N = 16
+    assert lib.lgv_em_create(C.byref(p), 0, 5, 0, C.byref(h)) == -1
+    p.noise_kind = 3
+    assert lib.lgv_em_create(C.byref(p), 10, 5, 0, C.byref(h)) == -1
+    assert b"noise_kind" in lib.lgv_em_last_error()
