@@ -5,7 +5,8 @@
                     [--graph] [--batched] [--drift] [--no-cpu-baseline] [--calibrate-copy]
 
 One "step" is PV2HBSplit::step(dt) (3 ideal substeps, 12 phi Metropolis
-half-sweeps, 6 charge sub-sweeps) applied to the whole lattice; the metric
+half-sweeps, 6 charge sub-sweeps; here 3 + 6 + 3 kernel launches) applied to
+the whole lattice; the metric
 is site-updates/s = sites x steps / wall seconds, inputs resident in HBM.
 Prints one JSON line.
 """
@@ -20,16 +21,16 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 # Algorithmic bytes per site per launch, FP64 fields (halve for FP32):
-# ideal substep (k_ideal_fused) read 80 + write 80; phi half-sweep
-# (k_hb_phi) read all phi 32 + write one parity 16; charge direction
-# (k_charge_dir, both sub-sweeps of one direction) read 48 + write 48.
-ALG_BYTES = {"ideal": 160, "phi": 48, "charge": 96}
-KERNEL = {"ideal": "k_ideal_fused", "phi": "k_hb_phi", "charge": "k_charge_dir"}
+# ideal substep (k_ideal_fused) read 80 + write 80; phi step (k_hb_fused,
+# both parities out of place) read 32 + write 32; charge direction
+# (k_charge_march, both sub-sweeps of one direction) read 48 + write 48.
+ALG_BYTES = {"ideal": 160, "phi": 64, "charge": 96}
+KERNEL = {"ideal": "k_ideal_fused", "phi": "k_hb_fused", "charge": "k_charge_march"}
 # Per site-update with one pass per dependency level of the reference
-# (six separate charge sub-sweeps): 3*160 + 12*48 + 6*96.
+# (12 phi half-sweeps of 48 B, 6 charge sub-sweeps of 96 B): 3*160 + 12*48 + 6*96.
 ALG_BYTES_PER_SITE_UPDATE = 1632
-# The same for this schedule, whose charge launches cover two sub-sweeps.
-SCHEDULE_BYTES_PER_SITE_UPDATE = 3 * 160 + 12 * 48 + 3 * 96  # 1344
+# The same for this schedule: 3 ideal, 6 phi and 3 charge launches.
+SCHEDULE_BYTES_PER_SITE_UPDATE = 3 * 160 + 6 * 64 + 3 * 96  # 1152
 # HBM bytes per launch per site measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (kernel, precision, N) -> (fetch, write), each scaled by the
 # copy-kernel calibration (bench.py --calibrate-copy); the summary of those
@@ -56,8 +57,16 @@ def calibrate_copy(mib=1024, reps=5):
     for _ in range(reps):
         torch.neg(x, out=y)
     torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        torch.neg(x, out=y)
+    t1.record()
+    torch.cuda.synchronize()
+    sec = t0.elapsed_time(t1) * 1e-3 / reps
     print(json.dumps({"calibration": "copy", "bytes_read": n * 8, "bytes_written": n * 8,
-                      "reps": reps, "kernel": "torch.neg (elementwise, FP64)"}), flush=True)
+                      "reps": 2 * reps, "kernel": "torch.neg (elementwise, FP64)",
+                      "copy_GBps": 2 * n * 8 / sec / 1e9}), flush=True)
 
 
 PARAMS = dict(mass0=-4.813, lam=1.0, H=0.01, chi=2.0, gamma=1.0)

@@ -37,7 +37,8 @@ enum { LGV_FP64 = 0, LGV_FP32 = 1 };
 
 /* ModelAData + ModelACoefficients + the seed of ModelAHandlerData
  * (ModelG/ModelA.h:71-111, 114-181, 185-248).  NY = NZ = NX as in the
- * reference; LX is read as an integer there (ModelA.h:229), 0 means LX = NX. */
+ * reference; LX is read as an integer there (ModelA.h:229), 0 means LX = NX.
+ * NX is even and at most 1536 (site indices are 32-bit on the device). */
 typedef struct {
   int NX;
   int LX;

@@ -89,7 +89,8 @@ struct lgv_ctx {
   ReplicaParams *dP = nullptr;
   ReplicaState *dS = nullptr;
   std::vector<void *> fields;
-  Plan plan_half, plan_full, plan_ideal;
+  Plan plan_half, plan_full, plan_ideal, plan_charge, plan_hb;
+  bool wide = false;  // some plane is >= 4 GiB: kernels use 64-bit offsets
   MonSlot *mon_phi = nullptr, *mon_charge = nullptr, *mon_ideal = nullptr;
   EnergySlot *eslots = nullptr;
   double coef_hb = NAN, coef_charge = NAN, coef_ideal = NAN;
@@ -111,7 +112,9 @@ struct lgv_ctx {
 
 namespace {
 
-int build_plan(Plan &plan, const std::vector<int64_t> &work) {
+int build_plan(Plan &plan, const std::vector<int64_t> &work, bool wide) {
+  plan.wide = wide;
+  plan.first.clear();
   std::vector<int> rep, first;
   for (size_t r = 0; r < work.size(); r++) {
     first.push_back((int)rep.size());
@@ -119,6 +122,7 @@ int build_plan(Plan &plan, const std::vector<int64_t> &work) {
     for (int64_t b = 0; b < nb; b++) rep.push_back((int)r);
   }
   plan.nblocks = (int)rep.size();
+  plan.first = first;
   HIPCHECK(hipMalloc(&plan.d_blk_rep, rep.size() * sizeof(int)));
   HIPCHECK(hipMalloc(&plan.d_blk_first, first.size() * sizeof(int)));
   HIPCHECK(hipMemcpy(plan.d_blk_rep, rep.data(), rep.size() * sizeof(int),
@@ -147,7 +151,35 @@ int build_ideal_plan(lgv_ctx *c) {
     HIPCHECK(hipFree(c->plan_ideal.d_blk_first));
     c->plan_ideal = Plan{};
   }
-  return build_plan(c->plan_ideal, work);
+  return build_plan(c->plan_ideal, work, c->wide);
+}
+
+// Blocks of the fused phi kernel: (x-tile, y-tile, z-chunk) per replica.
+int build_hb_plan(lgv_ctx *c) {
+  std::vector<int64_t> work;
+  for (int r = 0; r < c->nrep; r++) {
+    const ReplicaParams &p = c->hp[r];
+    int TXH, TY, TZ;
+    hb_tile(p.N, TXH, TY, TZ);
+    if ((TY + 2) * (2 * TXH + 2) > kHbLds || 2 * TY + 4 * TXH > kBlock)
+      return fail(LGV_ERR_ARG, "replica %d: phi tile %dx%d does not fit", r, TXH, TY);
+    work.push_back((int64_t)((p.NH + TXH - 1) / TXH) * ((p.N + TY - 1) / TY) *
+                   ((p.N + TZ - 1) / TZ) * kBlock);
+  }
+  return build_plan(c->plan_hb, work, c->wide);
+}
+
+// Blocks of the charge march kernel: (x-tile, line tile, chunk) per replica.
+int build_charge_plan(lgv_ctx *c) {
+  std::vector<int64_t> work;
+  for (int r = 0; r < c->nrep; r++) {
+    const ReplicaParams &p = c->hp[r];
+    int TXH, TQ, C;
+    charge_tile(p.N, TXH, TQ, C);
+    work.push_back((int64_t)((p.NH + TXH - 1) / TXH) * ((p.N + TQ - 1) / TQ) *
+                   ((p.N + C - 1) / C) * kBlock);
+  }
+  return build_plan(c->plan_charge, work, c->wide);
 }
 
 // Step coefficients with the reference's expressions:
@@ -236,11 +268,10 @@ void enqueue_ideal(lgv_ctx *c, int off) {
   }
 }
 
+// EulerLangevinHB::step: both parities (counters off, off+1) in one pass.
 void enqueue_hb(lgv_ctx *c, int off) {
-  for (int ieo = 0; ieo < 2; ieo++) {
-    Timed tm(c, LGV_CLASS_PHI);
-    launch_hb_phi(c->fp32, c->plan_half, c->dP, c->dS, ieo, off + ieo, c->mon_phi, c->stream);
-  }
+  Timed tm(c, LGV_CLASS_PHI);
+  launch_hb_fused(c->fp32, c->plan_hb, c->dP, c->dS, c->nrep, off, c->mon_phi, c->stream);
 }
 
 // ModelGChargeHB::step: the shuffle, then per direction both parities in
@@ -249,8 +280,8 @@ void enqueue_charge(lgv_ctx *c, int off) {
   launch_shuffle(c->nrep, c->dS, c->dP, off, c->stream);
   for (int ixyz = 0; ixyz < 3; ixyz++) {
     Timed tm(c, LGV_CLASS_CHARGE);
-    launch_charge_dir(c->fp32, c->plan_half, c->dP, c->dS, c->nrep, ixyz, off + 1 + 2 * ixyz,
-                      c->mon_charge, c->stream);
+    launch_charge_march(c->fp32, c->plan_charge, c->dP, c->dS, c->nrep, ixyz,
+                        off + 1 + 2 * ixyz, c->mon_charge, c->stream);
   }
 }
 
@@ -304,8 +335,8 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
   if (s0 > 64 || s1 > 64) return fail(LGV_ERR_ARG, "scounts too large");
   for (int r = 0; r < cfg->nreplicas; r++) {
     const lgv_model_params &m = cfg->replicas[r];
-    if (m.NX < 4 || (m.NX & 1) || m.NX > 2048)
-      return fail(LGV_ERR_ARG, "replica %d: NX=%d must be even and in [4, 2048]", r, m.NX);
+    if (m.NX < 4 || (m.NX & 1) || m.NX > kMaxN)
+      return fail(LGV_ERR_ARG, "replica %d: NX=%d must be even and in [4, %d]", r, m.NX, kMaxN);
     if (!(m.chi > 0)) return fail(LGV_ERR_ARG, "replica %d: chi must be positive", r);
   }
   std::unique_ptr<lgv_ctx> c(new lgv_ctx);
@@ -359,6 +390,7 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
     hs[r].order[3] = 0; hs[r].order[4] = 1;
     whalf.push_back(half);
     wfull.push_back(2 * half);
+    if ((uint64_t)half * esz > 0xFFFFFFFFull) c->wide = true;
     c->maxN = std::max(c->maxN, N);
   }
   HIPCHECK(hipMalloc(&c->dP, c->nrep * sizeof(ReplicaParams)));
@@ -367,13 +399,15 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
   HIPCHECK(hipMalloc(&c->dS, c->nrep * sizeof(ReplicaState)));
   HIPCHECK(hipMemcpy(c->dS, hs.data(), c->nrep * sizeof(ReplicaState), hipMemcpyHostToDevice));
   int rc;
-  if ((rc = build_plan(c->plan_half, whalf)) != LGV_OK) return rc;
-  if ((rc = build_plan(c->plan_full, wfull)) != LGV_OK) return rc;
+  if ((rc = build_plan(c->plan_half, whalf, c->wide)) != LGV_OK) return rc;
+  if ((rc = build_plan(c->plan_full, wfull, c->wide)) != LGV_OK) return rc;
   if ((rc = build_ideal_plan(c.get())) != LGV_OK) return rc;
-  HIPCHECK(hipMalloc(&c->mon_phi, c->plan_half.nblocks * sizeof(MonSlot)));
-  HIPCHECK(hipMemset(c->mon_phi, 0, c->plan_half.nblocks * sizeof(MonSlot)));
-  HIPCHECK(hipMalloc(&c->mon_charge, c->plan_half.nblocks * sizeof(MonSlot)));
-  HIPCHECK(hipMemset(c->mon_charge, 0, c->plan_half.nblocks * sizeof(MonSlot)));
+  if ((rc = build_charge_plan(c.get())) != LGV_OK) return rc;
+  if ((rc = build_hb_plan(c.get())) != LGV_OK) return rc;
+  HIPCHECK(hipMalloc(&c->mon_phi, c->plan_hb.nblocks * sizeof(MonSlot)));
+  HIPCHECK(hipMemset(c->mon_phi, 0, c->plan_hb.nblocks * sizeof(MonSlot)));
+  HIPCHECK(hipMalloc(&c->mon_charge, c->plan_charge.nblocks * sizeof(MonSlot)));
+  HIPCHECK(hipMemset(c->mon_charge, 0, c->plan_charge.nblocks * sizeof(MonSlot)));
   HIPCHECK(hipMalloc(&c->mon_ideal, c->nrep * sizeof(MonSlot)));
   HIPCHECK(hipMemset(c->mon_ideal, 0, c->nrep * sizeof(MonSlot)));
   HIPCHECK(hipMalloc(&c->eslots, c->plan_full.nblocks * sizeof(EnergySlot)));
@@ -399,7 +433,7 @@ int lgv_destroy(lgv_ctx *c) {
   for (void *f : c->fields) hipFree(f);
   hipFree(c->dP);
   hipFree(c->dS);
-  for (Plan *pl : {&c->plan_half, &c->plan_full, &c->plan_ideal}) {
+  for (Plan *pl : {&c->plan_half, &c->plan_full, &c->plan_ideal, &c->plan_charge, &c->plan_hb}) {
     hipFree(pl->d_blk_rep);
     hipFree(pl->d_blk_first);
   }
@@ -583,14 +617,12 @@ int lgv_monitor_get(lgv_ctx *c, int r, lgv_monitor out[3]) {
   if (rc) return rc;
   HIPCHECK(hipSetDevice(c->device));
   HIPCHECK(hipStreamSynchronize(c->stream));
-  std::vector<int> first(c->nrep);
-  HIPCHECK(hipMemcpy(first.data(), c->plan_half.d_blk_first, c->nrep * sizeof(int),
-                     hipMemcpyDeviceToHost));
-  const int b0 = first[r];
-  const int b1 = r + 1 < c->nrep ? first[r + 1] : c->plan_half.nblocks;
-  std::vector<MonSlot> slots(b1 - b0);
   MonSlot *src[2] = {c->mon_phi, c->mon_charge};
+  const Plan *plan[2] = {&c->plan_hb, &c->plan_charge};
   for (int k = 0; k < 2; k++) {
+    const int b0 = plan[k]->first[r];
+    const int b1 = r + 1 < c->nrep ? plan[k]->first[r + 1] : plan[k]->nblocks;
+    std::vector<MonSlot> slots(b1 - b0);
     HIPCHECK(hipMemcpy(slots.data(), src[k] + b0, slots.size() * sizeof(MonSlot),
                        hipMemcpyDeviceToHost));
     lgv_monitor m{};
@@ -835,8 +867,8 @@ int lgv_monitor_reset(lgv_ctx *c) {
   if (!c) return fail(LGV_ERR_ARG, "null context");
   HIPCHECK(hipSetDevice(c->device));
   HIPCHECK(hipStreamSynchronize(c->stream));
-  HIPCHECK(hipMemset(c->mon_phi, 0, c->plan_half.nblocks * sizeof(MonSlot)));
-  HIPCHECK(hipMemset(c->mon_charge, 0, c->plan_half.nblocks * sizeof(MonSlot)));
+  HIPCHECK(hipMemset(c->mon_phi, 0, c->plan_hb.nblocks * sizeof(MonSlot)));
+  HIPCHECK(hipMemset(c->mon_charge, 0, c->plan_charge.nblocks * sizeof(MonSlot)));
   HIPCHECK(hipMemset(c->mon_ideal, 0, c->nrep * sizeof(MonSlot)));
   HIPCHECK(hipDeviceSynchronize());
   return LGV_OK;

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace lgv {
 
 // Read-only (for the kernels) description of one replica.  The step
@@ -53,6 +55,28 @@ __host__ __device__ inline void ideal_tile(int N, int txh_cap, int tz, int &TXH,
   TZ = N < zc ? N : zc;
 }
 
+// Tile of the fused two-parity phi kernel: TXH half-columns x TY rows in
+// x-y, chunks of TZ planes in z; its LDS grid holds (TY+2)(2 TXH+2) sites.
+constexpr int kHbLds = 800;
+__host__ __device__ inline void hb_tile(int N, int &TXH, int &TY, int &TZ) {
+  const int NH = N / 2;
+  TXH = NH < 32 ? NH : 32;
+  TY = 256 / TXH;
+  if (TY > N) TY = N;
+  TZ = N < 16 ? N : 16;
+}
+
+// Tile of the charge march kernel: TXH half-columns x TQ lines of the
+// cross-section, chunks of C sites along the direction (C even).
+constexpr int kChargeChunk = 16;
+__host__ __device__ inline void charge_tile(int N, int &TXH, int &TQ, int &C) {
+  const int NH = N / 2;
+  TXH = NH < 64 ? NH : 64;
+  TQ = 256 / TXH;
+  if (TQ > N) TQ = N;
+  C = N < kChargeChunk ? N : kChargeChunk;
+}
+
 // Per-block monitor partials (o4_stepper_monitor counters).
 struct MonSlot {
   long long down_counts, up_counts, up_yes, up_no;
@@ -69,17 +93,26 @@ struct Plan {
   int nblocks = 0;
   int *d_blk_rep = nullptr;
   int *d_blk_first = nullptr;
+  bool wide = false;  // some plane of the launch is >= 4 GiB (64-bit offsets)
+  std::vector<int> first;  // host copy of d_blk_first
 };
+
+// Largest side: N^3 must fit 32-bit site indices.
+constexpr int kMaxN = 1536;
 
 constexpr int kBlock = 256;
 
 // Kernel launchers (lgv_kernels.hip).  fp32 selects the storage type.
 void launch_hb_phi(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
                    int ieo, int ctr_off, MonSlot *mon, hipStream_t st);
+void launch_hb_fused(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
+                     int nrep, int ctr_off, MonSlot *mon, hipStream_t st);
 void launch_shuffle(int nrep, ReplicaState *S, const ReplicaParams *P, int ctr_off,
                     hipStream_t st);
 void launch_charge_dir(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
                        int nrep, int ixyz, int ctr_off, MonSlot *mon, hipStream_t st);
+void launch_charge_march(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
+                         int nrep, int ixyz, int ctr_off, MonSlot *mon, hipStream_t st);
 void launch_ideal(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
                   int nrep, hipStream_t st);
 void launch_energy(bool fp32, const Plan &plan, const ReplicaParams *P,

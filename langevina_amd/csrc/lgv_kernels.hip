@@ -28,6 +28,33 @@ __device__ __forceinline__ int live(const ReplicaState &s) {
   return s.cur_phi | (s.cur_chg << 1);
 }
 
+// Global-memory view of one plane for the sweep kernels.  The base is
+// uniform over a block (it sits in SGPRs) and, with W == false, the element
+// offset is a 32-bit byte offset, so every access is one global load/store
+// in the saddr + 32-bit voffset form with no 64-bit address arithmetic.
+// W == true (a plane of 4 GiB or more) widens the offset to 64 bits.
+typedef __attribute__((address_space(1))) char gbyte;
+template <typename T, bool W>
+struct GPlane {
+  gbyte *b;
+  __device__ __forceinline__ static auto off(uint32_t i) {
+    if constexpr (W) return (uint64_t)i * sizeof(T);
+    else return (uint32_t)(i * (uint32_t)sizeof(T));
+  }
+  __device__ __forceinline__ double ld(uint32_t i) const {
+    return (double)*reinterpret_cast<const __attribute__((address_space(1))) T *>(b + off(i));
+  }
+  __device__ __forceinline__ void st(uint32_t i, double v) const {
+    *reinterpret_cast<__attribute__((address_space(1))) T *>(b + off(i)) = (T)v;
+  }
+};
+template <typename T, bool W>
+__device__ __forceinline__ GPlane<T, W> gplane(const ReplicaParams &p, int cur, int c, int par) {
+  const int buf = c < 4 ? (cur & 1) : (cur >> 1);
+  char *base = reinterpret_cast<char *>(buf ? p.fields[1] : p.fields[0]);
+  return {(gbyte *)(base + (int64_t)(c * 2 + par) * p.half * (int64_t)sizeof(T))};
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 // L2.  Renumber blocks so that XCD k runs the k-th contiguous eighth of the
 // plan: blocks that share neighbour planes then share an L2.
@@ -41,42 +68,84 @@ __device__ __forceinline__ int wrapm(int a, int N) { return a < 0 ? N - 1 : a; }
 
 // Deterministic block reduction of the monitor increments; thread 0 adds the
 // block total into the block's own slot (stream order makes this race-free).
-__device__ void block_monitor_add(MonSlot *slot, long long down, long long upyes,
-                                  long long upno, double dSd, double dSu) {
-  __shared__ long long sc[3][kBlock / 64];
+// The three counts travel packed in one 64-bit word, 21 bits each (a block
+// counts at most 12 decisions per thread).
+__device__ __forceinline__ uint64_t mon_pack(uint64_t down, uint64_t upyes, uint64_t upno) {
+  return down | (upyes << 21) | (upno << 42);
+}
+__device__ void block_monitor_add(MonSlot *slot, uint64_t cnt, double dSd, double dSu) {
+  __shared__ uint64_t sc[kBlock / 64];
   __shared__ double sd[2][kBlock / 64];
   for (int o = 32; o > 0; o >>= 1) {
-    down += __shfl_xor(down, o, 64);
-    upyes += __shfl_xor(upyes, o, 64);
-    upno += __shfl_xor(upno, o, 64);
+    cnt += __shfl_xor(cnt, o, 64);
     dSd += __shfl_xor(dSd, o, 64);
     dSu += __shfl_xor(dSu, o, 64);
   }
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
-    sc[0][w] = down; sc[1][w] = upyes; sc[2][w] = upno;
+    sc[w] = cnt;
     sd[0][w] = dSd; sd[1][w] = dSu;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    long long a = 0, b = 0, c = 0;
+    uint64_t c = 0;
     double d = 0, e = 0;
     for (int i = 0; i < kBlock / 64; i++) {
-      a += sc[0][i]; b += sc[1][i]; c += sc[2][i];
+      c += sc[i];
       d += sd[0][i]; e += sd[1][i];
     }
+    const long long a = (long long)(c & 0x1fffff), b = (long long)((c >> 21) & 0x1fffff),
+                    n = (long long)(c >> 42);
     slot->down_counts += a;
-    slot->up_counts += b + c;
+    slot->up_counts += b + n;
     slot->up_yes += b;
-    slot->up_no += c;
+    slot->up_no += n;
     slot->down_dS += d;
     slot->up_dS += e;
   }
 }
 
 // ------------------------------------------------------------------------
-// One parity of EulerLangevinHB::step (ModelG/Stepper.cxx:309-386).
-template <typename T>
+// The Metropolis decision of one phi site (Stepper.cxx:330-370) in the
+// reference's operand order: heff[l] is the neighbour sum of component l,
+// ((x+1 + x-1) + (y+1 + y-1)) + (z+1 + z-1).  Returns the new value (the
+// old one on reject) and adds the decision to the monitor when count.
+__device__ __forceinline__ void hb_decide(const double phi_o[4], const double heff[4],
+                                          double Lambda2, double lambda, double H, double rdtg,
+                                          Xoro &g, double phi_new[4], uint64_t &cnt,
+                                          double &dSd, double &dSu, bool count) {
+  double hphi_o = 0., hphi_n = 0., s_o = 0., s_n = 0.;
+  double phi_n[4];
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    phi_n[l] = phi_o[l] + rdtg * variance1(g);
+    s_o += phi_o[l] * phi_o[l];
+    s_n += phi_n[l] * phi_n[l];
+    hphi_o += heff[l] * phi_o[l];
+    hphi_n += heff[l] * phi_n[l];
+  }
+  hphi_o += (H * phi_o[0]);
+  hphi_n += (H * phi_n[0]);
+  const double dS = -(hphi_n - hphi_o) + Lambda2 * (s_n - s_o) +
+                    lambda / 4. * (s_n * s_n - s_o * s_o);
+  bool accept;
+  if (dS < 0) {
+    accept = true;
+    if (count) { cnt += mon_pack(1, 0, 0); dSd += dS; }
+  } else {
+    const double rr = uniform01(g);
+    accept = rr < exp(-dS);
+    if (count) {
+      if (accept) { cnt += mon_pack(0, 1, 0); dSu += dS; } else { cnt += mon_pack(0, 0, 1); }
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < 4; l++) phi_new[l] = accept ? phi_n[l] : phi_o[l];
+}
+
+// One parity of EulerLangevinHB::step (ModelG/Stepper.cxx:309-386), one
+// thread per site, in place.
+template <typename T, bool W>
 __global__ __launch_bounds__(kBlock) void k_hb_phi(const ReplicaParams *__restrict__ P,
                                                    const ReplicaState *__restrict__ S,
                                                    const int *__restrict__ blk_rep,
@@ -84,66 +153,220 @@ __global__ __launch_bounds__(kBlock) void k_hb_phi(const ReplicaParams *__restri
                                                    int ctr_off, MonSlot *__restrict__ mon) {
   const int bid = xcd_block();
   const int r = blk_rep[bid];
-  const ReplicaParams p = P[r];
-  const int64_t idx = (int64_t)(bid - blk_first[r]) * kBlock + threadIdx.x;
-  long long down = 0, upyes = 0, upno = 0;
+  const ReplicaParams &p = P[r];
+  const uint32_t idx = (uint32_t)(bid - blk_first[r]) * kBlock + threadIdx.x;
+  uint64_t cnt = 0;
   double dSd = 0., dSu = 0.;
-  if (idx < p.half) {
-    const int N = p.N, NH = p.NH;
-    const int xh = (int)(idx % NH);
-    const int64_t rowi = idx / NH;
-    const int y = (int)(rowi % N), z = (int)(rowi / N);
-    const int x = site_x(xh, y, z, ieo);
+  if (idx < (uint64_t)p.half) {
+    const uint32_t N = p.N, NH = p.NH;
+    const uint32_t rowi = idx / NH, xh = idx - rowi * NH;
+    const uint32_t z = rowi / N, y = rowi - z * N;
+    const uint32_t x = site_x(xh, y, z, ieo);
     const int q = ieo ^ 1;
-    const int64_t row = rowi * NH;
-    const int64_t nb[6] = {row + (wrapp(x + 1, N) >> 1), row + (wrapm(x - 1, N) >> 1),
-                           ((int64_t)z * N + wrapp(y + 1, N)) * NH + xh,
-                           ((int64_t)z * N + wrapm(y - 1, N)) * NH + xh,
-                           ((int64_t)wrapp(z + 1, N) * N + y) * NH + xh,
-                           ((int64_t)wrapm(z - 1, N) * N + y) * NH + xh};
-    const ReplicaState s = S[r];
+    const uint32_t row = rowi * NH;
+    const uint32_t xp = x + 1 == N ? 0 : x + 1, xm = x == 0 ? N - 1 : x - 1;
+    const uint32_t yp = y + 1 == N ? 0 : y + 1, ym = y == 0 ? N - 1 : y - 1;
+    const uint32_t zp = z + 1 == N ? 0 : z + 1, zm = z == 0 ? N - 1 : z - 1;
+    const uint32_t nb[6] = {row + (xp >> 1), row + (xm >> 1), (z * N + yp) * NH + xh,
+                            (z * N + ym) * NH + xh, (zp * N + y) * NH + xh,
+                            (zm * N + y) * NH + xh};
+    const ReplicaState &s = S[r];
+    const int cur = live(s);
     const double mass = p.mass0 + p.dmassdt * s.t;
     const double Lambda2 = 0.5 * (2. * 3. + mass);
-    const double lambda = p.lambda;
-    const double H = p.H;
-    const double rdtg = p.rdtg;
     Xoro g = xoro_seeded(site_key(stream_key(p.base_key, s.sweep_ctr + ctr_off),
-                                  ((uint64_t)z * N + y) * N + x));
-    double hphi_o = 0., hphi_n = 0., s_o = 0., s_n = 0.;
-    double phi_o[4], phi_n[4];
+                                  (uint64_t)rowi * N + x));
+    double phi_o[4], heff[4], phi_n[4];
 #pragma unroll
     for (int l = 0; l < 4; l++) {
-      const T *fq = plane<T>(p, live(s), l, q);
-      const double heff = ((double)fq[nb[0]] + (double)fq[nb[1]]) +
-                          ((double)fq[nb[2]] + (double)fq[nb[3]]) +
-                          ((double)fq[nb[4]] + (double)fq[nb[5]]);
-      phi_o[l] = (double)plane<T>(p, live(s), l, ieo)[idx];
-      phi_n[l] = phi_o[l] + rdtg * variance1(g);
-      s_o += phi_o[l] * phi_o[l];
-      s_n += phi_n[l] * phi_n[l];
-      hphi_o += heff * phi_o[l];
-      hphi_n += heff * phi_n[l];
+      const GPlane<T, W> fq = gplane<T, W>(p, cur, l, q);
+      heff[l] = (fq.ld(nb[0]) + fq.ld(nb[1])) + (fq.ld(nb[2]) + fq.ld(nb[3])) +
+                (fq.ld(nb[4]) + fq.ld(nb[5]));
+      phi_o[l] = gplane<T, W>(p, cur, l, ieo).ld(idx);
     }
-    hphi_o += (H * phi_o[0]);
-    hphi_n += (H * phi_n[0]);
-    const double dS = -(hphi_n - hphi_o) + Lambda2 * (s_n - s_o) +
-                      lambda / 4. * (s_n * s_n - s_o * s_o);
-    bool accept;
-    if (dS < 0) {
-      accept = true;
-      down = 1;
-      dSd = dS;
-    } else {
-      const double rr = uniform01(g);
-      accept = rr < exp(-dS);
-      if (accept) { upyes = 1; dSu = dS; } else { upno = 1; }
-    }
-    if (accept) {
+    hb_decide(phi_o, heff, Lambda2, p.lambda, p.H, p.rdtg, g, phi_n, cnt, dSd, dSu, true);
 #pragma unroll
-      for (int l = 0; l < 4; l++) plane<T>(p, live(s), l, ieo)[idx] = (T)phi_n[l];
+    for (int l = 0; l < 4; l++) gplane<T, W>(p, cur, l, ieo).st(idx, phi_n[l]);
+  }
+  block_monitor_add(mon + bid, cnt, dSd, dSu);
+}
+
+// Both parities of EulerLangevinHB::step in one pass, out of place (reads
+// the live phi buffer, writes the other).  A block owns a TXH x TY x-y tile
+// of half-columns (one site of each parity per plane) and marches z through
+// its chunk, one plane behind for the second parity: at plane z it updates
+// the parity-0 sites of z, then the parity-1 sites of z-1, whose parity-0
+// neighbours are then all final.  An LDS grid per plane holds the tile and a
+// one-site ring: the parity-1 entries carry old values (in-plane neighbours
+// of the parity-0 update), the parity-0 entries new values (in-plane
+// neighbours of the parity-1 update one plane later); two grids alternate.
+// z-neighbours at the thread's own (x, y) stay in registers.  Ring parity-0
+// sites are recomputed from global memory with their own counter streams,
+// so they carry the same bits as in the block that owns them; the monitor
+// counts each decision once, in the owning block.
+__device__ __forceinline__ int wrapz(int z, int N) { return z < 0 ? z + N : (z >= N ? z - N : z); }
+
+template <typename T, bool W>
+__global__ __launch_bounds__(kBlock) void k_hb_fused(const ReplicaParams *__restrict__ P,
+                                                     const ReplicaState *__restrict__ S,
+                                                     const int *__restrict__ blk_rep,
+                                                     const int *__restrict__ blk_first,
+                                                     int ctr_off, MonSlot *__restrict__ mon) {
+  __shared__ T G[2][4][kHbLds];
+  const int bid = xcd_block();
+  const int r = blk_rep[bid];
+  const ReplicaParams &p = P[r];
+  const ReplicaState &s = S[r];
+  const int cur = live(s), nxt = cur ^ 1;
+  const int N = p.N, NH = p.NH;
+  int TXH, TY, TZ;
+  hb_tile(N, TXH, TY, TZ);
+  const int ntx = (NH + TXH - 1) / TXH, nty = (N + TY - 1) / TY;
+  int b = bid - blk_first[r];
+  const int bx = b % ntx;
+  b /= ntx;
+  const int by = b % nty, bz = b / nty;
+  const int xh0 = bx * TXH, y0 = by * TY, z0 = bz * TZ;
+  const int tw = min(TXH, NH - xh0), th = min(TY, N - y0), tz = min(TZ, N - z0);
+  const int LW = 2 * tw + 2;  // LDS row width
+  const int t = threadIdx.x;
+  const int tx = t % TXH, ty = t / TXH;
+  const bool own = t < TXH * TY && ty < th && tx < tw;
+  const int xh = xh0 + tx, y = y0 + ty;
+  // ring site of this thread (one-site ring around the tile, no corners)
+  const bool halo = t < 2 * th + 4 * tw;
+  int hlx = 0, hly = 0;
+  if (t < th) { hlx = -1; hly = t; }
+  else if (t < 2 * th) { hlx = 2 * tw; hly = t - th; }
+  else if (t < 2 * th + 2 * tw) { hlx = t - 2 * th; hly = -1; }
+  else { hlx = t - 2 * th - 2 * tw; hly = th; }
+  const int hgx = wrapz(2 * xh0 + hlx, N), hgy = wrapz(y0 + hly, N);
+  const int hpos = (hly + 1) * LW + (hlx + 1);
+
+  const double mass = p.mass0 + p.dmassdt * s.t;
+  const double Lambda2 = 0.5 * (2. * 3. + mass);
+  const double lambda = p.lambda, H = p.H, rdtg = p.rdtg;
+  const uint64_t key0 = stream_key(p.base_key, s.sweep_ctr + ctr_off);
+  const uint64_t key1 = stream_key(p.base_key, s.sweep_ctr + ctr_off + 1);
+  uint64_t cnt = 0;
+  double dSd = 0., dSu = 0.;
+
+  auto idx_of = [&](int x, int yy, int z) __attribute__((always_inline)) {
+    return ((uint32_t)z * N + yy) * NH + (x >> 1);
+  };
+  auto ld4 = [&](int par, uint32_t i, double v[4]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int l = 0; l < 4; l++) v[l] = gplane<T, W>(p, cur, l, par).ld(i);
+  };
+  // x of this thread's site of parity par in plane z
+  auto own_x = [&](int par, int z) __attribute__((always_inline)) {
+    return 2 * xh + ((par + y + z) & 1);
+  };
+
+  // own half-column: old parity-1 values at z-1, z, z+1; new parity-0 values
+  // at z-2, z-1, z
+  double o1m[4], o1c[4], o1p[4], n0m2[4] = {}, n0m1[4] = {}, n0c[4] = {};
+  if (own) {
+    const int za = wrapz(z0 - 2, N), zb = wrapz(z0 - 1, N);
+    ld4(1, idx_of(own_x(1, za), y, za), o1m);
+    ld4(1, idx_of(own_x(1, zb), y, zb), o1c);
+  }
+  for (int j = 0; j < tz + 2; j++) {
+    const int zu = z0 - 1 + j;
+    const int z = wrapz(zu, N);
+    const int zp = z + 1 == N ? 0 : z + 1;
+    const bool inchunk = zu >= z0 && zu < z0 + tz;
+    T(*g)[kHbLds] = G[j & 1];
+    double o0c[4];
+    if (own) {
+      const int x0 = own_x(0, z), x1 = own_x(1, z);
+      ld4(1, idx_of(x1, y, zp), o1p);
+      ld4(0, idx_of(x0, y, z), o0c);
+      const int pos = (ty + 1) * LW + (x1 - 2 * xh0 + 1);
+#pragma unroll
+      for (int l = 0; l < 4; l++) g[l][pos] = (T)o1c[l];
+    }
+    if (halo) {
+      if (((hgx + hgy + z) & 1) == 1) {
+        double v[4];
+        ld4(1, idx_of(hgx, hgy, z), v);
+#pragma unroll
+        for (int l = 0; l < 4; l++) g[l][hpos] = (T)v[l];
+      } else if (inchunk) {
+        // parity-0 ring site: the update its owner computes, from global memory
+        const int xp = hgx + 1 == N ? 0 : hgx + 1, xm = hgx == 0 ? N - 1 : hgx - 1;
+        const int yp = hgy + 1 == N ? 0 : hgy + 1, ym = hgy == 0 ? N - 1 : hgy - 1;
+        const int zm = z == 0 ? N - 1 : z - 1;
+        const uint32_t nb[6] = {idx_of(xp, hgy, z), idx_of(xm, hgy, z), idx_of(hgx, yp, z),
+                                idx_of(hgx, ym, z), idx_of(hgx, hgy, zp), idx_of(hgx, hgy, zm)};
+        double phi_o[4], heff[4], phi_n[4];
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+          const GPlane<T, W> fq = gplane<T, W>(p, cur, l, 1);
+          heff[l] = (fq.ld(nb[0]) + fq.ld(nb[1])) + (fq.ld(nb[2]) + fq.ld(nb[3])) +
+                    (fq.ld(nb[4]) + fq.ld(nb[5]));
+        }
+        ld4(0, idx_of(hgx, hgy, z), phi_o);
+        Xoro gg = xoro_seeded(site_key(key0, ((uint64_t)z * N + hgy) * N + hgx));
+        hb_decide(phi_o, heff, Lambda2, lambda, H, rdtg, gg, phi_n, cnt, dSd, dSu, false);
+#pragma unroll
+        for (int l = 0; l < 4; l++) g[l][hpos] = (T)phi_n[l];
+      }
+    }
+    __syncthreads();
+    if (own) {
+      // parity 0 at plane z
+      const int x0 = own_x(0, z);
+      const int pos = (ty + 1) * LW + (x0 - 2 * xh0 + 1);
+      double heff[4];
+#pragma unroll
+      for (int l = 0; l < 4; l++)
+        heff[l] = ((double)g[l][pos + 1] + (double)g[l][pos - 1]) +
+                  ((double)g[l][pos + LW] + (double)g[l][pos - LW]) + (o1p[l] + o1m[l]);
+      Xoro gg = xoro_seeded(site_key(key0, ((uint64_t)z * N + y) * N + x0));
+      hb_decide(o0c, heff, Lambda2, lambda, H, rdtg, gg, n0c, cnt, dSd, dSu, inchunk);
+#pragma unroll
+      for (int l = 0; l < 4; l++) {
+        n0c[l] = (double)(T)n0c[l];
+        g[l][pos] = (T)n0c[l];
+      }
+      // parity 1 at plane z-1, from the other grid
+      const int zbu = zu - 1;
+      if (zbu >= z0 && zbu < z0 + tz) {
+        const int zb = wrapz(zbu, N);
+        T(*gb)[kHbLds] = G[(j + 1) & 1];
+        const int x1 = own_x(1, zb);
+        const int pb = (ty + 1) * LW + (x1 - 2 * xh0 + 1);
+#pragma unroll
+        for (int l = 0; l < 4; l++)
+          heff[l] = ((double)gb[l][pb + 1] + (double)gb[l][pb - 1]) +
+                    ((double)gb[l][pb + LW] + (double)gb[l][pb - LW]) + (n0c[l] + n0m2[l]);
+        Xoro g1 = xoro_seeded(site_key(key1, ((uint64_t)zb * N + y) * N + x1));
+        double n1[4];
+        hb_decide(o1m, heff, Lambda2, lambda, H, rdtg, g1, n1, cnt, dSd, dSu, true);
+        const uint32_t i = idx_of(x1, y, zb);
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+          gplane<T, W>(p, nxt, l, 1).st(i, n1[l]);
+          gplane<T, W>(p, nxt, l, 0).st(i, n0m1[l]);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+      o1m[l] = o1c[l];
+      o1c[l] = o1p[l];
+      n0m2[l] = n0m1[l];
+      n0m1[l] = n0c[l];
     }
   }
-  block_monitor_add(mon + bid, down, upyes, upno, dSd, dSu);
+  block_monitor_add(mon + bid, cnt, dSd, dSu);
+}
+
+__global__ void k_flip_phi(ReplicaState *S, int nrep) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < nrep) S[r].cur_phi ^= 1;
 }
 
 // ------------------------------------------------------------------------
@@ -169,21 +392,19 @@ __global__ void k_shuffle(ReplicaState *S, const ReplicaParams *P, int nrep, int
 // counter streams, identical bits); the monitor counts it once, from the
 // thread on its right.  Reads the live charge buffer, writes the other one.
 __device__ __forceinline__ void charge_pair(double nA, double nB, double rms, double chi,
-                                            Xoro &g, double &outA, double &outB,
-                                            long long &down, long long &upyes,
-                                            long long &upno, double &dSd, double &dSu,
-                                            bool count) {
+                                            Xoro &g, double &outA, double &outB, uint64_t &cnt,
+                                            double &dSd, double &dSu, bool count) {
   double q = rms * variance1(g);
   const double dS = ((nA - q) * (nA - q) + (nB + q) * (nB + q) - nA * nA - nB * nB) /
                     (2. * chi);
   if (dS < 0) {
-    if (count) { down++; dSd += dS; }
+    if (count) { cnt += mon_pack(1, 0, 0); dSd += dS; }
   } else {
     const double rr = uniform01(g);
     if (rr < exp(-dS)) {
-      if (count) { upyes++; dSu += dS; }
+      if (count) { cnt += mon_pack(0, 1, 0); dSu += dS; }
     } else {
-      if (count) upno++;
+      if (count) cnt += mon_pack(0, 0, 1);
       q = 0.;
     }
   }
@@ -191,7 +412,54 @@ __device__ __forceinline__ void charge_pair(double nA, double nB, double rms, do
   outB = nB + q;
 }
 
-template <typename T>
+// One A' thread of the in-line scheme (see k_charge_dir) for direction dir.
+template <typename T, bool W>
+__device__ __forceinline__ void charge_line(const ReplicaParams &p, int cur, int nxt, int dir,
+                                            int e1, int e2, uint32_t xh, uint32_t y, uint32_t z,
+                                            uint64_t k1, uint64_t k2, uint64_t &cnt,
+                                            double &dSd, double &dSu) {
+  const uint32_t N = p.N, NH = p.NH;
+  const uint32_t x = site_x(xh, y, z, e2);
+  // the four sites along the direction: L, A', B', R (parities e1 e2 e1 e2)
+  const uint32_t a0 = dir == 0 ? x : (dir == 1 ? y : z);
+  const uint32_t am = a0 == 0 ? N - 1 : a0 - 1;
+  const uint32_t ap = a0 + 1 == N ? 0 : a0 + 1;
+  const uint32_t app = ap + 1 == N ? 0 : ap + 1;
+  uint32_t ii[4], lin[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint32_t a = k == 0 ? am : (k == 1 ? a0 : (k == 2 ? ap : app));
+    const uint32_t xx = dir == 0 ? a : x, yy = dir == 1 ? a : y, zz = dir == 2 ? a : z;
+    ii[k] = (zz * N + yy) * NH + (xx >> 1);
+    lin[k] = (zz * N + yy) * N + xx;
+  }
+  const double chi = p.chi, rms = p.rms;
+  Xoro gL = xoro_seeded(site_key(k1, lin[0]));
+  Xoro gB = xoro_seeded(site_key(k1, lin[2]));
+  Xoro gA = xoro_seeded(site_key(k2, lin[1]));
+  uint64_t c0 = 0;
+  double sd0 = 0., su0 = 0.;
+#pragma unroll
+  for (int L = 4; L < 10; L++) {
+    const GPlane<T, W> f1 = gplane<T, W>(p, cur, L, e1), f2 = gplane<T, W>(p, cur, L, e2);
+    const double vL = f1.ld(ii[0]);
+    const double vA = f2.ld(ii[1]);
+    const double vB = f1.ld(ii[2]);
+    const double vR = f2.ld(ii[3]);
+    double tL, a1, b1, tR;
+    charge_pair(vL, vA, rms, chi, gL, tL, a1, cnt, dSd, dSu, true);
+    charge_pair(vB, vR, rms, chi, gB, b1, tR, c0, sd0, su0, false);
+    // the first sub-sweep's results as stored (rounded in FP32 storage)
+    a1 = (double)(T)a1;
+    b1 = (double)(T)b1;
+    double a2, b2;
+    charge_pair(a1, b1, rms, chi, gA, a2, b2, cnt, dSd, dSu, true);
+    gplane<T, W>(p, nxt, L, e2).st(ii[1], a2);
+    gplane<T, W>(p, nxt, L, e1).st(ii[2], b2);
+  }
+}
+
+template <typename T, bool W>
 __global__ __launch_bounds__(kBlock) void k_charge_dir(const ReplicaParams *__restrict__ P,
                                                        const ReplicaState *__restrict__ S,
                                                        const int *__restrict__ blk_rep,
@@ -200,61 +468,138 @@ __global__ __launch_bounds__(kBlock) void k_charge_dir(const ReplicaParams *__re
                                                        MonSlot *__restrict__ mon) {
   const int bid = xcd_block();
   const int r = blk_rep[bid];
-  const ReplicaParams p = P[r];
-  const int64_t idx = (int64_t)(bid - blk_first[r]) * kBlock + threadIdx.x;
-  long long down = 0, upyes = 0, upno = 0;
+  const ReplicaParams &p = P[r];
+  const uint32_t idx = (uint32_t)(bid - blk_first[r]) * kBlock + threadIdx.x;
+  uint64_t cnt = 0;
   double dSd = 0., dSu = 0.;
-  if (idx < p.half) {
-    const ReplicaState s = S[r];
+  if (idx < (uint64_t)p.half) {
+    const ReplicaState &s = S[r];
     const int cur = live(s), nxt = cur ^ 2;
-    const int dir = S[r].order[ixyz];
-    const int e1 = S[r].order[3], e2 = S[r].order[4];
-    const int N = p.N, NH = p.NH;
-    const int xh = (int)(idx % NH);
-    const int64_t rowi = idx / NH;
-    const int y = (int)(rowi % N), z = (int)(rowi / N);
-    const int x = site_x(xh, y, z, e2);
-    // the four sites along the direction: L, A', B', R (parities e1 e2 e1 e2)
-    const int a0 = dir == 0 ? x : (dir == 1 ? y : z);
-    const int am = a0 == 0 ? N - 1 : a0 - 1;
-    const int ap = a0 + 1 == N ? 0 : a0 + 1;
-    const int app = ap + 1 == N ? 0 : ap + 1;
-    int64_t ii[4];
-    uint64_t lin[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int a = k == 0 ? am : (k == 1 ? a0 : (k == 2 ? ap : app));
-      const int xx = dir == 0 ? a : x, yy = dir == 1 ? a : y, zz = dir == 2 ? a : z;
-      ii[k] = ((int64_t)zz * N + yy) * NH + (xx >> 1);
-      lin[k] = ((uint64_t)zz * N + yy) * N + xx;
-    }
-    const double chi = p.chi, rms = p.rms;
+    const int dir = s.order[ixyz];
+    const int e1 = s.order[3], e2 = s.order[4];
+    const uint32_t N = p.N, NH = p.NH;
+    const uint32_t rowi = idx / NH, xh = idx - rowi * NH;
+    const uint32_t z = rowi / N, y = rowi - z * N;
     const uint64_t k1 = stream_key(p.base_key, s.sweep_ctr + ctr_off);
     const uint64_t k2 = stream_key(p.base_key, s.sweep_ctr + ctr_off + 1);
-    Xoro gL = xoro_seeded(site_key(k1, lin[0]));
-    Xoro gB = xoro_seeded(site_key(k1, lin[2]));
-    Xoro gA = xoro_seeded(site_key(k2, lin[1]));
-    long long d0 = 0, y0 = 0, n0 = 0;
-    double sd0 = 0., su0 = 0.;
+    charge_line<T, W>(p, cur, nxt, dir, e1, e2, xh, y, z, k1, k2, cnt, dSd, dSu);
+  }
+  block_monitor_add(mon + bid, cnt, dSd, dSu);
+}
+
+template <typename T, bool W>
+__device__ __forceinline__ void charge_row_x(const ReplicaParams &p, int cur, int nxt, int e1,
+                                             int e2, uint32_t xh, uint32_t y, uint32_t z,
+                                             uint64_t k1, uint64_t k2, uint64_t &cnt,
+                                             double &dSd, double &dSu) {
+  charge_line<T, W>(p, cur, nxt, 0, e1, e2, xh, y, z, k1, k2, cnt, dSd, dSu);
+}
+
+// The same two sub-sweeps with every site loaded once and every pair
+// computed once (except one first-parity pair per chunk end).  Blocks tile
+// the lattice as TXH half-columns x TQ lines x a chunk of kChargeChunk sites:
+//  * d = y or z: a thread owns the two columns x = 2xh, 2xh+1 along d and
+//    marches d through its chunk.  Along a column the parity alternates; at
+//    each step it updates the next first-parity pair (m, m+1) and then the
+//    second-parity pair (m-1, m) from the previous pair's upper site, kept in
+//    registers, and this pair's lower site.  The chunk owns the
+//    second-parity pairs whose lower site lies in it and the first-parity
+//    pairs whose lower site lies in it (the monitor counts those).
+//  * d = x: pairs lie in rows; the thread runs the in-row scheme of
+//    k_charge_dir for the row (y, z) of each z of its chunk.
+template <typename T, bool W>
+__device__ __forceinline__ void charge_column(const ReplicaParams &p, int cur, int nxt, int dir,
+                                              int e1, int e2, uint32_t x, uint32_t q, int c0,
+                                              int c1, uint64_t k1, uint64_t k2, uint64_t &cnt,
+                                              double &dSd, double &dSu) {
+  const int N = p.N, NH = p.NH;
+  const double chi = p.chi, rms = p.rms;
+  // index / linear site of position m along the column
+  auto at = [&](int m, uint32_t &ii, uint32_t &lin) __attribute__((always_inline)) {
+    const uint32_t row = dir == 2 ? (uint32_t)m * N + q : q * N + (uint32_t)m;
+    ii = row * NH + (x >> 1);
+    lin = row * N + x;
+  };
+  // lower site of the first first-parity pair: c0-1 or c0
+  const int mm0 = (int)((x + q + (uint32_t)(c0 + N - 1)) & 1) == e1 ? c0 - 1 : c0;
+  double up[6];  // upper site of the previous first-parity pair
+  const int npair = (c1 - c0) / 2;
+  for (int k = 0; k <= npair; k++) {
+    const int mm = mm0 + 2 * k;
+    const int m1 = mm < 0 ? mm + N : (mm >= N ? mm - N : mm);
+    const int m2 = m1 + 1 == N ? 0 : m1 + 1;
+    uint32_t i1, l1, i2, l2;
+    at(m1, i1, l1);
+    at(m2, i2, l2);
+    const bool own1 = mm >= c0 && mm < c1;
+    Xoro g1 = xoro_seeded(site_key(k1, l1));
+    Xoro g2;
+    uint32_t i0 = 0, l0;
+    if (k > 0) {
+      const int m0 = m1 == 0 ? N - 1 : m1 - 1;
+      at(m0, i0, l0);
+      g2 = xoro_seeded(site_key(k2, l0));
+    }
 #pragma unroll
     for (int L = 4; L < 10; L++) {
-      const double vL = (double)plane<T>(p, cur, L, e1)[ii[0]];
-      const double vA = (double)plane<T>(p, cur, L, e2)[ii[1]];
-      const double vB = (double)plane<T>(p, cur, L, e1)[ii[2]];
-      const double vR = (double)plane<T>(p, cur, L, e2)[ii[3]];
-      double tL, a1, b1, tR;
-      charge_pair(vL, vA, rms, chi, gL, tL, a1, down, upyes, upno, dSd, dSu, true);
-      charge_pair(vB, vR, rms, chi, gB, b1, tR, d0, y0, n0, sd0, su0, false);
-      // the first sub-sweep's results as stored (rounded in FP32 storage)
-      a1 = (double)(T)a1;
-      b1 = (double)(T)b1;
-      double a2, b2;
-      charge_pair(a1, b1, rms, chi, gA, a2, b2, down, upyes, upno, dSd, dSu, true);
-      plane<T>(p, nxt, L, e2)[ii[1]] = (T)a2;
-      plane<T>(p, nxt, L, e1)[ii[2]] = (T)b2;
+      const double v1 = gplane<T, W>(p, cur, L, e1).ld(i1);
+      const double v2 = gplane<T, W>(p, cur, L, e2).ld(i2);
+      double a, b;
+      charge_pair(v1, v2, rms, chi, g1, a, b, cnt, dSd, dSu, own1);
+      a = (double)(T)a;
+      b = (double)(T)b;
+      if (k > 0) {
+        double o0, o1;
+        charge_pair(up[L - 4], a, rms, chi, g2, o0, o1, cnt, dSd, dSu, true);
+        gplane<T, W>(p, nxt, L, e2).st(i0, o0);
+        gplane<T, W>(p, nxt, L, e1).st(i1, o1);
+      }
+      up[L - 4] = b;
     }
   }
-  block_monitor_add(mon + bid, down, upyes, upno, dSd, dSu);
+}
+
+template <typename T, bool W>
+__global__ __launch_bounds__(kBlock) void k_charge_march(const ReplicaParams *__restrict__ P,
+                                                         const ReplicaState *__restrict__ S,
+                                                         const int *__restrict__ blk_rep,
+                                                         const int *__restrict__ blk_first,
+                                                         int ixyz, int ctr_off,
+                                                         MonSlot *__restrict__ mon) {
+  const int bid = xcd_block();
+  const int r = blk_rep[bid];
+  const ReplicaParams &p = P[r];
+  const ReplicaState &s = S[r];
+  const int N = p.N, NH = p.NH;
+  int TXH, TQ, C;
+  charge_tile(N, TXH, TQ, C);
+  const int nbx = (NH + TXH - 1) / TXH, nbq = (N + TQ - 1) / TQ;
+  int b = bid - blk_first[r];
+  const int bx = b % nbx;
+  b /= nbx;
+  const int bq = b % nbq, bc = b / nbq;
+  const int t = threadIdx.x;
+  const uint32_t xh = bx * TXH + t % TXH, q = bq * TQ + t / TXH;
+  const int c0 = bc * C, c1 = min(c0 + C, N);
+  uint64_t cnt = 0;
+  double dSd = 0., dSu = 0.;
+  if (xh < (uint32_t)NH && q < (uint32_t)N && t < TXH * TQ) {
+    const int cur = live(s), nxt = cur ^ 2;
+    const int dir = s.order[ixyz];
+    const int e1 = s.order[3], e2 = s.order[4];
+    const uint64_t k1 = stream_key(p.base_key, s.sweep_ctr + ctr_off);
+    const uint64_t k2 = stream_key(p.base_key, s.sweep_ctr + ctr_off + 1);
+    if (dir == 0) {
+      for (int z = c0; z < c1; z++)
+        charge_row_x<T, W>(p, cur, nxt, e1, e2, xh, q, (uint32_t)z, k1, k2, cnt, dSd, dSu);
+    } else {
+#pragma unroll 1
+      for (int sx = 0; sx < 2; sx++)
+        charge_column<T, W>(p, cur, nxt, dir, e1, e2, 2 * xh + sx, q, c0, c1, k1, k2, cnt, dSd,
+                            dSu);
+    }
+  }
+  block_monitor_add(mon + bid, cnt, dSd, dSu);
 }
 
 __global__ void k_flip_chg(ReplicaState *S, int nrep) {
@@ -270,21 +615,23 @@ __global__ void k_flip_chg(ReplicaState *S, int nrep) {
 // of the current plane and a one-site x-y halo (recomputed from the raw
 // fields) sit in LDS, the rotated phi of the planes below and above in
 // registers.  Reads the live buffer, writes the other one.
-template <typename T>
+template <typename T, bool W>
 __device__ __forceinline__ void load_site(const ReplicaParams &p, int cur, int x, int y, int z,
                                           double f[10]) {
   const int par = (x + y + z) & 1;
-  const int64_t i = ((int64_t)z * p.N + y) * p.NH + (x >> 1);
+  const uint32_t i = ((uint32_t)z * p.N + y) * p.NH + (x >> 1);
 #pragma unroll
-  for (int c = 0; c < 10; c++) f[c] = (double)plane<T>(p, cur, c, par)[i];
+  for (int c = 0; c < 10; c++) f[c] = gplane<T, W>(p, cur, c, par).ld(i);
 }
 
-template <typename T>
+template <typename T, bool W>
 __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__restrict__ P,
                                                         const ReplicaState *__restrict__ S,
                                                         const int *__restrict__ blk_rep,
                                                         const int *__restrict__ blk_first) {
   __shared__ double lds[4][kIdealLds];
+  // rotated phi of plane z-1 at the thread's own sites: thread-private slots
+  __shared__ double lprv[2][4][kBlock];
   const int bid = xcd_block();
   const int r = blk_rep[bid];
   // a reference, not a copy: a private copy would be staged in scratch memory
@@ -300,7 +647,7 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
   const int by = b % nty, bz = b / nty;
   const int xh0 = bx * TXH, y0 = by * TY, z0 = bz * TZ;
   const int tw = min(TXH, NH - xh0), th = min(TY, N - y0), tz = min(TZ, N - z0);
-  const int W = 2 * tw + 2;
+  const int LW = 2 * tw + 2;  // LDS row width
   const int t = threadIdx.x;
   const int tx = t % TXH, ty = t / TXH;
   const bool own = ty < th && tx < tw;
@@ -317,22 +664,19 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
   const double h = 0.5 * (p.tau_ideal / p.chi);
   const double axx = p.axx, H0 = p.H, dt = p.dt_ideal;
 
-  double prv[2][4], cen[2][4], nex[2][4];  // rotated phi, own sites, z-1, z, z+1
-  double chg[2][6], chn[2][6];             // raw charges of planes z and z+1
-  auto rotate_own = [&](int z, double out[2][4], double ch[2][6]) __attribute__((always_inline)) {
+  double cen[2][4], nex[2][4];  // rotated phi at the own sites of planes z, z+1
+  auto rotate_own = [&](int z, double out[2][4]) __attribute__((always_inline)) {
     if (!own) return;
-    const int64_t i = ((int64_t)z * N + y) * NH + xh;
+    const uint32_t i = ((uint32_t)z * N + y) * NH + xh;
 #pragma unroll
     for (int s = 0; s < 2; s++) {
       const int par = (s + y + z) & 1;
       double f[10];
 #pragma unroll
-      for (int c = 0; c < 10; c++) f[c] = (double)plane<T>(p, cur, c, par)[i];
+      for (int c = 0; c < 10; c++) f[c] = gplane<T, W>(p, cur, c, par).ld(i);
       rotate_site_fast(f, f + 4, f + 7, h);
 #pragma unroll
       for (int c = 0; c < 4; c++) out[s][c] = f[c];
-#pragma unroll
-      for (int c = 0; c < 6; c++) ch[s][c] = f[4 + c];
     }
   };
   auto fill_lds = [&](int z, const double o[2][4]) __attribute__((always_inline)) {
@@ -340,28 +684,33 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
 #pragma unroll
       for (int s = 0; s < 2; s++)
 #pragma unroll
-        for (int c = 0; c < 4; c++) lds[c][(ty + 1) * W + (2 * tx + s + 1)] = o[s][c];
+        for (int c = 0; c < 4; c++) lds[c][(ty + 1) * LW + (2 * tx + s + 1)] = o[s][c];
     }
     if (halo) {
       double f[10];
-      load_site<T>(p, cur, hgx, hgy, z, f);
+      load_site<T, W>(p, cur, hgx, hgy, z, f);
       rotate_site_fast(f, f + 4, f + 7, h);
 #pragma unroll
-      for (int c = 0; c < 4; c++) lds[c][(hly + 1) * W + (hlx + 1)] = f[c];
+      for (int c = 0; c < 4; c++) lds[c][(hly + 1) * LW + (hlx + 1)] = f[c];
     }
   };
 
-  double dummy[2][6];
-  rotate_own((z0 - 1 + N) % N, prv, dummy);
-  rotate_own(z0, cen, chg);
+  rotate_own((z0 - 1 + N) % N, nex);
+  if (own) {
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) lprv[s][c][t] = nex[s][c];
+  }
+  rotate_own(z0, cen);
   fill_lds(z0, cen);
   for (int k = 0; k < tz; k++) {
     const int z = z0 + k;
     const int zn = (z + 1) % N;
-    rotate_own(zn, nex, chn);
+    rotate_own(zn, nex);
     __syncthreads();
     if (own) {
-      const int64_t i = ((int64_t)z * N + y) * NH + xh;
+      const uint32_t i = ((uint32_t)z * N + y) * NH + xh;
 #pragma unroll
       for (int s = 0; s < 2; s++) {
         const int lx = 2 * tx + s + 1, ly = ty + 1;
@@ -369,17 +718,21 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
 #pragma unroll
         for (int c = 0; c < 4; c++) {
           cp[c] = cen[s][c];
-          xp[c] = lds[c][ly * W + lx + 1];
-          xm[c] = lds[c][ly * W + lx - 1];
-          yp[c] = lds[c][(ly + 1) * W + lx];
-          ym[c] = lds[c][(ly - 1) * W + lx];
+          xp[c] = lds[c][ly * LW + lx + 1];
+          xm[c] = lds[c][ly * LW + lx - 1];
+          yp[c] = lds[c][(ly + 1) * LW + lx];
+          ym[c] = lds[c][(ly - 1) * LW + lx];
         }
-        const double *zp = nex[s], *zm = prv[s];
-        double A[3], V[3];
+        const int par = (s + y + z) & 1;
+        const double *zp = nex[s];
+        double zm[4], A[3], V[3];
+#pragma unroll
+        for (int c = 0; c < 4; c++) zm[c] = lprv[s][c][t];
+        // the charges of plane z, unchanged by the first rotation
 #pragma unroll
         for (int l = 0; l < 3; l++) {
-          A[l] = chg[s][l];
-          V[l] = chg[s][3 + l];
+          A[l] = gplane<T, W>(p, cur, 4 + l, par).ld(i);
+          V[l] = gplane<T, W>(p, cur, 7 + l, par).ld(i);
         }
 #pragma unroll
         for (int l = 0; l < 3; l++) {
@@ -403,13 +756,12 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
           V[s3] += dt * (advxx + advyy + advzz);
         }
         rotate_site_fast(cp, A, V, h);
-        const int par = (s + y + z) & 1;
 #pragma unroll
-        for (int c = 0; c < 4; c++) plane<T>(p, nxt, c, par)[i] = (T)cp[c];
+        for (int c = 0; c < 4; c++) gplane<T, W>(p, nxt, c, par).st(i, cp[c]);
 #pragma unroll
-        for (int c = 0; c < 3; c++) plane<T>(p, nxt, 4 + c, par)[i] = (T)A[c];
+        for (int c = 0; c < 3; c++) gplane<T, W>(p, nxt, 4 + c, par).st(i, A[c]);
 #pragma unroll
-        for (int c = 0; c < 3; c++) plane<T>(p, nxt, 7 + c, par)[i] = (T)V[c];
+        for (int c = 0; c < 3; c++) gplane<T, W>(p, nxt, 7 + c, par).st(i, V[c]);
       }
     }
     __syncthreads();
@@ -418,11 +770,9 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
     for (int s = 0; s < 2; s++) {
 #pragma unroll
       for (int c = 0; c < 4; c++) {
-        prv[s][c] = cen[s][c];
+        if (own) lprv[s][c][t] = cen[s][c];
         cen[s][c] = nex[s][c];
       }
-#pragma unroll
-      for (int c = 0; c < 6; c++) chg[s][c] = chn[s][c];
     }
   }
 }
@@ -719,14 +1069,29 @@ void launch_wall_dft(int N, const double *wall, const double *wallphase, double 
 
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// W: some plane of the launch is 4 GiB or larger (64-bit element offsets).
+#define LGV_DISPATCH(KERNEL, GRID, ST, ...)                                           \
+  do {                                                                                \
+    if (fp32) {                                                                       \
+      if (plan.wide) KERNEL<float, true><<<GRID, kBlock, 0, ST>>>(__VA_ARGS__);       \
+      else KERNEL<float, false><<<GRID, kBlock, 0, ST>>>(__VA_ARGS__);                \
+    } else {                                                                          \
+      if (plan.wide) KERNEL<double, true><<<GRID, kBlock, 0, ST>>>(__VA_ARGS__);      \
+      else KERNEL<double, false><<<GRID, kBlock, 0, ST>>>(__VA_ARGS__);               \
+    }                                                                                 \
+  } while (0)
+
 void launch_hb_phi(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
                    int ieo, int ctr_off, MonSlot *mon, hipStream_t st) {
-  if (fp32)
-    k_hb_phi<float><<<plan.nblocks, kBlock, 0, st>>>(P, S, plan.d_blk_rep, plan.d_blk_first,
-                                                     ieo, ctr_off, mon);
-  else
-    k_hb_phi<double><<<plan.nblocks, kBlock, 0, st>>>(P, S, plan.d_blk_rep, plan.d_blk_first,
-                                                      ieo, ctr_off, mon);
+  LGV_DISPATCH(k_hb_phi, plan.nblocks, st, P, S, plan.d_blk_rep, plan.d_blk_first, ieo,
+               ctr_off, mon);
+}
+
+void launch_hb_fused(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
+                     int nrep, int ctr_off, MonSlot *mon, hipStream_t st) {
+  LGV_DISPATCH(k_hb_fused, plan.nblocks, st, P, S, plan.d_blk_rep, plan.d_blk_first, ctr_off,
+               mon);
+  k_flip_phi<<<nblk(nrep, 64), 64, 0, st>>>(S, nrep);
 }
 
 void launch_shuffle(int nrep, ReplicaState *S, const ReplicaParams *P, int ctr_off,
@@ -736,23 +1101,21 @@ void launch_shuffle(int nrep, ReplicaState *S, const ReplicaParams *P, int ctr_o
 
 void launch_charge_dir(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
                        int nrep, int ixyz, int ctr_off, MonSlot *mon, hipStream_t st) {
-  if (fp32)
-    k_charge_dir<float><<<plan.nblocks, kBlock, 0, st>>>(P, S, plan.d_blk_rep,
-                                                         plan.d_blk_first, ixyz, ctr_off, mon);
-  else
-    k_charge_dir<double><<<plan.nblocks, kBlock, 0, st>>>(P, S, plan.d_blk_rep,
-                                                          plan.d_blk_first, ixyz, ctr_off, mon);
+  LGV_DISPATCH(k_charge_dir, plan.nblocks, st, P, S, plan.d_blk_rep, plan.d_blk_first, ixyz,
+               ctr_off, mon);
+  k_flip_chg<<<nblk(nrep, 64), 64, 0, st>>>(S, nrep);
+}
+
+void launch_charge_march(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
+                         int nrep, int ixyz, int ctr_off, MonSlot *mon, hipStream_t st) {
+  LGV_DISPATCH(k_charge_march, plan.nblocks, st, P, S, plan.d_blk_rep, plan.d_blk_first, ixyz,
+               ctr_off, mon);
   k_flip_chg<<<nblk(nrep, 64), 64, 0, st>>>(S, nrep);
 }
 
 void launch_ideal(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
                   int nrep, hipStream_t st) {
-  if (fp32)
-    k_ideal_fused<float><<<plan.nblocks, kBlock, 0, st>>>(P, S, plan.d_blk_rep,
-                                                          plan.d_blk_first);
-  else
-    k_ideal_fused<double><<<plan.nblocks, kBlock, 0, st>>>(P, S, plan.d_blk_rep,
-                                                           plan.d_blk_first);
+  LGV_DISPATCH(k_ideal_fused, plan.nblocks, st, P, S, plan.d_blk_rep, plan.d_blk_first);
   k_flip<<<nblk(nrep, 64), 64, 0, st>>>(S, nrep);
 }
 
