@@ -74,25 +74,37 @@ DT = 0.04
 
 
 def cpu_baseline(seconds_target=12.0):
-    """The CPU oracle (a C restatement of the reference's stepper, one
-    thread) timed on this host on a bounded sample: 128^3 FP64 PV2HBSplit23
-    steps until ~seconds_target of work."""
+    """The CPU oracle (a C restatement of the reference's stepper) timed on
+    this host on a bounded sample: 128^3 FP64 PV2HBSplit23 steps until
+    ~seconds_target of work, its loops split over z-planes across the host's
+    cores (at most 16, the CPU share of one GPU), as MPI ranks split the
+    reference's lattice."""
     from oracle import oracle as orc
     N = 128
+    cores = max(1, min(16, os.cpu_count() or 1))
+    orc.set_threads(cores)
     lat = orc.Lattice(N, mass0=PARAMS["mass0"], lam=PARAMS["lam"], H=PARAMS["H"],
                       chi=PARAMS["chi"], seed=42, rng_mode=orc.RNG_PER_SITE)
     lat.init_gaussian_charges()
+    lat.step(DT)  # first touch of the pages and thread start-up
     steps, t0 = 0, time.perf_counter()
     while True:
         lat.step(DT)
         steps += 1
         el = time.perf_counter() - t0
-        if el > seconds_target or steps >= 10:
+        if el > seconds_target or steps >= 40:
             break
-    return {"value": N ** 3 * steps / el, "unit": "site-updates/s", "cores": 1,
-            "kind": "port",
+    orc.set_threads(1)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"value": N ** 3 * steps / el, "unit": "site-updates/s", "cores": cores,
+            "kind": "port", "host": f"{model}, nproc {os.cpu_count()}",
             "sample": f"oracle/lgv_oracle.c, {steps} PV2HBSplit23 step(s) of a {N}^3 FP64 "
-                      f"lattice ({el:.1f} s, 1 thread)"}
+                      f"lattice after one untimed step ({el:.1f} s, {cores} OpenMP threads)"}
 
 
 def batched_replicas(L):
@@ -281,6 +293,13 @@ def main():
     tm = ctx.timing()
     ctx.timing_enable(False)
 
+    # the measurement pass (Measurer::measure), reported separately
+    ctx.measure_full()
+    tm0 = time.perf_counter()
+    for _ in range(5):
+        ctx.measure_full()
+    meas_s = (time.perf_counter() - tm0) / 5
+
     sites = N ** 3
     value = sites * args.steps / el
     ms_per_step = el * 1e3 / args.steps
@@ -319,6 +338,9 @@ def main():
                      "frac": achieved / HBM_PEAK, "traffic": traffic(dom, args.precision, N),
                      "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": per_launch_s * 1e6},
         "kernels": kernels,
+        "measurement": {"ms": meas_s * 1e3, "alg_GBps": N ** 3 * 10 * esz / meas_s / 1e9,
+                        "what": "lgv_measure_full: walls, phase walls, averages, device r2c "
+                                "of every row, rotate_to_vev; host copies included"},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()

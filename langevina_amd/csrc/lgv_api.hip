@@ -142,6 +142,9 @@ int build_ideal_plan(lgv_ctx *c) {
     ideal_tile(p.N, p.tile_cap, p.tile_tz, TXH, TY, TZ);
     if ((TY + 2) * (2 * TXH + 2) > kIdealLds)
       return fail(LGV_ERR_ARG, "replica %d: ideal tile %dx%d exceeds the LDS plane", r, TXH, TY);
+    if (2 * TY + 4 * TXH > kBlock)
+      return fail(LGV_ERR_ARG, "replica %d: ideal tile %dx%d has a halo ring of %d sites, "
+                  "more than the %d threads of a block", r, TXH, TY, 2 * TY + 4 * TXH, kBlock);
     work.push_back((int64_t)((p.NH + TXH - 1) / TXH) * ((p.N + TY - 1) / TY) *
                    ((p.N + TZ - 1) / TZ) * kBlock);
   }
@@ -268,10 +271,21 @@ void enqueue_ideal(lgv_ctx *c, int off) {
   }
 }
 
-// EulerLangevinHB::step: both parities (counters off, off+1) in one pass.
+// EulerLangevinHB::step (counters off, off+1 for the two parities).  FP64
+// fields: both parities in one out-of-place pass; FP32 fields: one in-place
+// pass per parity, measured faster there (the fused pass is bound by its
+// arithmetic and barriers, which FP32 storage does not shrink).
+const Plan &hb_plan(const lgv_ctx *c) { return c->fp32 ? c->plan_half : c->plan_hb; }
 void enqueue_hb(lgv_ctx *c, int off) {
-  Timed tm(c, LGV_CLASS_PHI);
-  launch_hb_fused(c->fp32, c->plan_hb, c->dP, c->dS, c->nrep, off, c->mon_phi, c->stream);
+  if (c->fp32) {
+    for (int ieo = 0; ieo < 2; ieo++) {
+      Timed tm(c, LGV_CLASS_PHI);
+      launch_hb_phi(true, c->plan_half, c->dP, c->dS, ieo, off + ieo, c->mon_phi, c->stream);
+    }
+  } else {
+    Timed tm(c, LGV_CLASS_PHI);
+    launch_hb_fused(false, c->plan_hb, c->dP, c->dS, c->nrep, off, c->mon_phi, c->stream);
+  }
 }
 
 // ModelGChargeHB::step: the shuffle, then per direction both parities in
@@ -404,8 +418,8 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
   if ((rc = build_ideal_plan(c.get())) != LGV_OK) return rc;
   if ((rc = build_charge_plan(c.get())) != LGV_OK) return rc;
   if ((rc = build_hb_plan(c.get())) != LGV_OK) return rc;
-  HIPCHECK(hipMalloc(&c->mon_phi, c->plan_hb.nblocks * sizeof(MonSlot)));
-  HIPCHECK(hipMemset(c->mon_phi, 0, c->plan_hb.nblocks * sizeof(MonSlot)));
+  HIPCHECK(hipMalloc(&c->mon_phi, hb_plan(c.get()).nblocks * sizeof(MonSlot)));
+  HIPCHECK(hipMemset(c->mon_phi, 0, hb_plan(c.get()).nblocks * sizeof(MonSlot)));
   HIPCHECK(hipMalloc(&c->mon_charge, c->plan_charge.nblocks * sizeof(MonSlot)));
   HIPCHECK(hipMemset(c->mon_charge, 0, c->plan_charge.nblocks * sizeof(MonSlot)));
   HIPCHECK(hipMalloc(&c->mon_ideal, c->nrep * sizeof(MonSlot)));
@@ -618,7 +632,7 @@ int lgv_monitor_get(lgv_ctx *c, int r, lgv_monitor out[3]) {
   HIPCHECK(hipSetDevice(c->device));
   HIPCHECK(hipStreamSynchronize(c->stream));
   MonSlot *src[2] = {c->mon_phi, c->mon_charge};
-  const Plan *plan[2] = {&c->plan_hb, &c->plan_charge};
+  const Plan *plan[2] = {&hb_plan(c), &c->plan_charge};
   for (int k = 0; k < 2; k++) {
     const int b0 = plan[k]->first[r];
     const int b1 = r + 1 < c->nrep ? plan[k]->first[r + 1] : plan[k]->nblocks;
@@ -867,7 +881,7 @@ int lgv_monitor_reset(lgv_ctx *c) {
   if (!c) return fail(LGV_ERR_ARG, "null context");
   HIPCHECK(hipSetDevice(c->device));
   HIPCHECK(hipStreamSynchronize(c->stream));
-  HIPCHECK(hipMemset(c->mon_phi, 0, c->plan_hb.nblocks * sizeof(MonSlot)));
+  HIPCHECK(hipMemset(c->mon_phi, 0, hb_plan(c).nblocks * sizeof(MonSlot)));
   HIPCHECK(hipMemset(c->mon_charge, 0, c->plan_charge.nblocks * sizeof(MonSlot)));
   HIPCHECK(hipMemset(c->mon_ideal, 0, c->nrep * sizeof(MonSlot)));
   HIPCHECK(hipDeviceSynchronize());

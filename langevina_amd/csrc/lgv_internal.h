@@ -109,8 +109,6 @@ void launch_hb_fused(bool fp32, const Plan &plan, const ReplicaParams *P, Replic
                      int nrep, int ctr_off, MonSlot *mon, hipStream_t st);
 void launch_shuffle(int nrep, ReplicaState *S, const ReplicaParams *P, int ctr_off,
                     hipStream_t st);
-void launch_charge_dir(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
-                       int nrep, int ixyz, int ctr_off, MonSlot *mon, hipStream_t st);
 void launch_charge_march(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
                          int nrep, int ixyz, int ctr_off, MonSlot *mon, hipStream_t st);
 void launch_ideal(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,

@@ -55,13 +55,10 @@ __device__ __forceinline__ GPlane<T, W> gplane(const ReplicaParams &p, int cur, 
   return {(gbyte *)(base + (int64_t)(c * 2 + par) * p.half * (int64_t)sizeof(T))};
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
-// L2.  Renumber blocks so that XCD k runs the k-th contiguous eighth of the
-// plan: blocks that share neighbour planes then share an L2.
-__device__ __forceinline__ int xcd_block() {
-  const int b = blockIdx.x, n = gridDim.x, q = n >> 3;
-  return b < (q << 3) ? (b & 7) * q + (b >> 3) : b;
-}
+// Block index of this workgroup within the launch plan.  (A renumbering that
+// gives each XCD a contiguous eighth of the plan was measured on MI355X and
+// made the sweeps slower; blocks run in plain dispatch order.)
+__device__ __forceinline__ int xcd_block() { return blockIdx.x; }
 
 __device__ __forceinline__ int wrapp(int a, int N) { return a == N ? 0 : a; }
 __device__ __forceinline__ int wrapm(int a, int N) { return a < 0 ? N - 1 : a; }
@@ -382,15 +379,8 @@ __global__ void k_shuffle(ReplicaState *S, const ReplicaParams *P, int nrep, int
 }
 
 // ------------------------------------------------------------------------
-// The two sub-sweeps of one direction of ModelGChargeHB::step (the shuffled
-// order always runs both parities of a direction back to back,
-// Stepper.cxx:471-473) in one pass.  Thread = site A' of the second parity
-// e2; along the direction its line reads L = A'-e, A', B' = A'+e, R = A'+2e.
-// First sub-sweep (parity e1): pairs (L, A') and (B', R), keyed by L and B';
-// second: pair (A', B') keyed by A' on the first sub-sweep's results.  Each
-// first-parity pair is computed by the two threads that need it (identical
-// counter streams, identical bits); the monitor counts it once, from the
-// thread on its right.  Reads the live charge buffer, writes the other one.
+// modelg_update_charge_pair (Stepper.cxx:419-441) for one component of one
+// pair, applied as nA + (-q), nB + q; counts the decision when count.
 __device__ __forceinline__ void charge_pair(double nA, double nB, double rms, double chi,
                                             Xoro &g, double &outA, double &outB, uint64_t &cnt,
                                             double &dSd, double &dSu, bool count) {
@@ -412,7 +402,15 @@ __device__ __forceinline__ void charge_pair(double nA, double nB, double rms, do
   outB = nB + q;
 }
 
-// One A' thread of the in-line scheme (see k_charge_dir) for direction dir.
+// The two sub-sweeps of one direction (the shuffled order always runs both
+// parities of a direction back to back, Stepper.cxx:471-473) for one site
+// A' of the second parity e2: along the direction its line reads
+// L = A'-e, A', B' = A'+e, R = A'+2e.  First sub-sweep (parity e1): pairs
+// (L, A') and (B', R), keyed by L and B'; second: pair (A', B') keyed by A'
+// on the first sub-sweep's results.  Each first-parity pair is computed by
+// the two threads that need it (identical counter streams, identical bits);
+// the monitor counts it once, from the thread on its right.  Reads the live
+// charge buffer, writes A' and B' into the other one.
 template <typename T, bool W>
 __device__ __forceinline__ void charge_line(const ReplicaParams &p, int cur, int nxt, int dir,
                                             int e1, int e2, uint32_t xh, uint32_t y, uint32_t z,
@@ -459,42 +457,6 @@ __device__ __forceinline__ void charge_line(const ReplicaParams &p, int cur, int
   }
 }
 
-template <typename T, bool W>
-__global__ __launch_bounds__(kBlock) void k_charge_dir(const ReplicaParams *__restrict__ P,
-                                                       const ReplicaState *__restrict__ S,
-                                                       const int *__restrict__ blk_rep,
-                                                       const int *__restrict__ blk_first,
-                                                       int ixyz, int ctr_off,
-                                                       MonSlot *__restrict__ mon) {
-  const int bid = xcd_block();
-  const int r = blk_rep[bid];
-  const ReplicaParams &p = P[r];
-  const uint32_t idx = (uint32_t)(bid - blk_first[r]) * kBlock + threadIdx.x;
-  uint64_t cnt = 0;
-  double dSd = 0., dSu = 0.;
-  if (idx < (uint64_t)p.half) {
-    const ReplicaState &s = S[r];
-    const int cur = live(s), nxt = cur ^ 2;
-    const int dir = s.order[ixyz];
-    const int e1 = s.order[3], e2 = s.order[4];
-    const uint32_t N = p.N, NH = p.NH;
-    const uint32_t rowi = idx / NH, xh = idx - rowi * NH;
-    const uint32_t z = rowi / N, y = rowi - z * N;
-    const uint64_t k1 = stream_key(p.base_key, s.sweep_ctr + ctr_off);
-    const uint64_t k2 = stream_key(p.base_key, s.sweep_ctr + ctr_off + 1);
-    charge_line<T, W>(p, cur, nxt, dir, e1, e2, xh, y, z, k1, k2, cnt, dSd, dSu);
-  }
-  block_monitor_add(mon + bid, cnt, dSd, dSu);
-}
-
-template <typename T, bool W>
-__device__ __forceinline__ void charge_row_x(const ReplicaParams &p, int cur, int nxt, int e1,
-                                             int e2, uint32_t xh, uint32_t y, uint32_t z,
-                                             uint64_t k1, uint64_t k2, uint64_t &cnt,
-                                             double &dSd, double &dSu) {
-  charge_line<T, W>(p, cur, nxt, 0, e1, e2, xh, y, z, k1, k2, cnt, dSd, dSu);
-}
-
 // The same two sub-sweeps with every site loaded once and every pair
 // computed once (except one first-parity pair per chunk end).  Blocks tile
 // the lattice as TXH half-columns x TQ lines x a chunk of kChargeChunk sites:
@@ -505,8 +467,8 @@ __device__ __forceinline__ void charge_row_x(const ReplicaParams &p, int cur, in
 //    registers, and this pair's lower site.  The chunk owns the
 //    second-parity pairs whose lower site lies in it and the first-parity
 //    pairs whose lower site lies in it (the monitor counts those).
-//  * d = x: pairs lie in rows; the thread runs the in-row scheme of
-//    k_charge_dir for the row (y, z) of each z of its chunk.
+//  * d = x: pairs lie in rows; the thread runs charge_line for the row
+//    (y, z) of each z of its chunk.
 template <typename T, bool W>
 __device__ __forceinline__ void charge_column(const ReplicaParams &p, int cur, int nxt, int dir,
                                               int e1, int e2, uint32_t x, uint32_t q, int c0,
@@ -591,7 +553,7 @@ __global__ __launch_bounds__(kBlock) void k_charge_march(const ReplicaParams *__
     const uint64_t k2 = stream_key(p.base_key, s.sweep_ctr + ctr_off + 1);
     if (dir == 0) {
       for (int z = c0; z < c1; z++)
-        charge_row_x<T, W>(p, cur, nxt, e1, e2, xh, q, (uint32_t)z, k1, k2, cnt, dSd, dSu);
+        charge_line<T, W>(p, cur, nxt, 0, e1, e2, xh, q, (uint32_t)z, k1, k2, cnt, dSd, dSu);
     } else {
 #pragma unroll 1
       for (int sx = 0; sx < 2; sx++)
@@ -613,8 +575,9 @@ __global__ void k_flip_chg(ReplicaState *S, int nrep) {
 // dt/2 with the kicked charges.  A block owns a TXH x TY x-y tile (both
 // parities of each half-column) and marches over TZ planes; the rotated phi
 // of the current plane and a one-site x-y halo (recomputed from the raw
-// fields) sit in LDS, the rotated phi of the planes below and above in
-// registers.  Reads the live buffer, writes the other one.
+// fields) sit in LDS, the rotated phi of the plane above in registers and of
+// the plane below in thread-private LDS slots.  Reads the live buffer,
+// writes the other one.
 template <typename T, bool W>
 __device__ __forceinline__ void load_site(const ReplicaParams &p, int cur, int x, int y, int z,
                                           double f[10]) {
@@ -664,8 +627,14 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
   const double h = 0.5 * (p.tau_ideal / p.chi);
   const double axx = p.axx, H0 = p.H, dt = p.dt_ideal;
 
+  // FP64 fields carry the raw charges of planes z and z+1 in registers; FP32
+  // fields re-read those of plane z at kick time, which keeps the kernel at
+  // three waves per SIMD (measured faster for FP32, slower for FP64, whose
+  // re-reads miss L2)
+  constexpr bool kCarry = sizeof(T) == 8;
   double cen[2][4], nex[2][4];  // rotated phi at the own sites of planes z, z+1
-  auto rotate_own = [&](int z, double out[2][4]) __attribute__((always_inline)) {
+  double chg[2][6], chn[2][6];  // raw charges of planes z, z+1 (kCarry only)
+  auto rotate_own = [&](int z, double out[2][4], double ch[2][6]) __attribute__((always_inline)) {
     if (!own) return;
     const uint32_t i = ((uint32_t)z * N + y) * NH + xh;
 #pragma unroll
@@ -677,6 +646,10 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
       rotate_site_fast(f, f + 4, f + 7, h);
 #pragma unroll
       for (int c = 0; c < 4; c++) out[s][c] = f[c];
+      if constexpr (kCarry) {
+#pragma unroll
+        for (int c = 0; c < 6; c++) ch[s][c] = f[4 + c];
+      }
     }
   };
   auto fill_lds = [&](int z, const double o[2][4]) __attribute__((always_inline)) {
@@ -695,19 +668,19 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
     }
   };
 
-  rotate_own((z0 - 1 + N) % N, nex);
+  rotate_own((z0 - 1 + N) % N, nex, chn);
   if (own) {
 #pragma unroll
     for (int s = 0; s < 2; s++)
 #pragma unroll
       for (int c = 0; c < 4; c++) lprv[s][c][t] = nex[s][c];
   }
-  rotate_own(z0, cen);
+  rotate_own(z0, cen, chg);
   fill_lds(z0, cen);
   for (int k = 0; k < tz; k++) {
     const int z = z0 + k;
     const int zn = (z + 1) % N;
-    rotate_own(zn, nex);
+    rotate_own(zn, nex, chn);
     __syncthreads();
     if (own) {
       const uint32_t i = ((uint32_t)z * N + y) * NH + xh;
@@ -731,8 +704,13 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
         // the charges of plane z, unchanged by the first rotation
 #pragma unroll
         for (int l = 0; l < 3; l++) {
-          A[l] = gplane<T, W>(p, cur, 4 + l, par).ld(i);
-          V[l] = gplane<T, W>(p, cur, 7 + l, par).ld(i);
+          if constexpr (kCarry) {
+            A[l] = chg[s][l];
+            V[l] = chg[s][3 + l];
+          } else {
+            A[l] = gplane<T, W>(p, cur, 4 + l, par).ld(i);
+            V[l] = gplane<T, W>(p, cur, 7 + l, par).ld(i);
+          }
         }
 #pragma unroll
         for (int l = 0; l < 3; l++) {
@@ -772,6 +750,10 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
       for (int c = 0; c < 4; c++) {
         if (own) lprv[s][c][t] = cen[s][c];
         cen[s][c] = nex[s][c];
+      }
+      if constexpr (kCarry) {
+#pragma unroll
+        for (int c = 0; c < 6; c++) chg[s][c] = chn[s][c];
       }
     }
   }
@@ -1097,13 +1079,6 @@ void launch_hb_fused(bool fp32, const Plan &plan, const ReplicaParams *P, Replic
 void launch_shuffle(int nrep, ReplicaState *S, const ReplicaParams *P, int ctr_off,
                     hipStream_t st) {
   k_shuffle<<<nblk(nrep, 64), 64, 0, st>>>(S, P, nrep, ctr_off);
-}
-
-void launch_charge_dir(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
-                       int nrep, int ixyz, int ctr_off, MonSlot *mon, hipStream_t st) {
-  LGV_DISPATCH(k_charge_dir, plan.nblocks, st, P, S, plan.d_blk_rep, plan.d_blk_first, ixyz,
-               ctr_off, mon);
-  k_flip_chg<<<nblk(nrep, 64), 64, 0, st>>>(S, nrep);
 }
 
 void launch_charge_march(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,

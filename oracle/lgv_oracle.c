@@ -22,8 +22,7 @@
  *                         energy drift second order in dt;
  *   phi Metropolis     -> free-field closed form 1/(m^2+k^2) (freeCase.ipynb
  *                         cell 7 free2pt), detailed balance on a 2-site
- *                         brute-force enumeration, ensemble value of
- *                         hcrit1_N032_m-04813_h010000_l10000_averages_ave.out;
+ *                         brute-force enumeration;
  *   charge Metropolis  -> exact conservation of sum n, equilibrium variance
  *                         chi, transition-kernel histogram against the
  *                         closed-form Metropolis kernel;
@@ -32,12 +31,22 @@
  *   Euler--Maruyama    -> free forward-Euler two-point function
  *                         (freeCase.ipynb cells 5, 7 free2ptFE) and central
  *                         finite differences for the adjoint.
+ *
+ * orc_set_threads(n > 1) runs the per-site-stream mode's sweeps, rotations,
+ * kick and energy sums over n OpenMP threads (z-planes split, as MPI ranks
+ * split the reference's lattice).  Fields and accept/reject decisions are
+ * unchanged; the monitor and energy sums are added in another order.  Only
+ * the CPU throughput baseline uses it.
  */
 #include "lgv_oracle.h"
 
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+
+static int g_threads = 1;
+void orc_set_threads(int n) { g_threads = n > 1 ? n : 1; }
+int orc_get_threads(void) { return g_threads; }
 
 /* ======================================================================
  * Random numbers
@@ -348,6 +357,8 @@ double orc_energy(const orc_ctx *c, const double *u) {
   const int N = c->N;
   const double H[4] = {c->H, 0., 0., 0.};
   double grad2 = 0, nab2 = 0, hEn = 0;
+  const int nt = c->rng_mode == ORC_RNG_PER_SITE ? g_threads : 1;
+#pragma omp parallel for num_threads(nt) if (nt > 1) reduction(+ : grad2, nab2, hEn) schedule(static)
   for (int k = 0; k < N; k++)
     for (int j = 0; j < N; j++)
       for (int i = 0; i < N; i++) {
@@ -376,6 +387,8 @@ double orc_energy(const orc_ctx *c, const double *u) {
 /* IdealPV2::rotatePhi, ModelG/Stepper.cxx:178-202 */
 void orc_rotate_phi(const orc_ctx *c, double *u, double tau) {
   const size_t V = (size_t)c->N * c->N * c->N;
+  const int nt = c->rng_mode == ORC_RNG_PER_SITE ? g_threads : 1;
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static)
   for (size_t s = 0; s < V; s++) {
     double *p = u + s * NDOF;
     double vmu[3], amu[3];
@@ -392,10 +405,13 @@ void orc_ideal_kick(const orc_ctx *c, double *u, double dt) {
   const int N = c->N;
   const size_t V = (size_t)N * N * N;
   double *f = (double *)malloc(V * 4 * sizeof(double));
+  const int nt = c->rng_mode == ORC_RNG_PER_SITE ? g_threads : 1;
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static)
   for (size_t s = 0; s < V; s++) memcpy(f + 4 * s, u + NDOF * s, 4 * sizeof(double));
   const double hinv = 1. / c->hx;
   const double axx = hinv * hinv, ayy = axx, azz = axx; /* pow(1/h,2) */
   const double H0 = c->H;
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static)
   for (int k = 0; k < N; k++)
     for (int j = 0; j < N; j++)
       for (int i = 0; i < N; i++) {
@@ -458,6 +474,58 @@ void orc_ideal_step(orc_ctx *c, double *u, double dt) {
   }
 }
 
+/* One site of EulerLangevinHB::step, ModelG/Stepper.cxx:330-383 */
+static inline void hb_site(orc_ctx *c, double *u, int k, int j, int i, uint64_t skey,
+                           double Lambda2, double lambda, double H, double rdtg,
+                           orc_monitor *mon) {
+  const int N = c->N;
+  draw_src d;
+  orc_xoro *g = site_engine(c, &d, skey, site_index(N, k, j, i));
+  double *p = u + NDOF * site_index(N, k, j, i);
+  const double *xp = u + NDOF * site_index(N, k, j, wrap(i + 1, N));
+  const double *xm = u + NDOF * site_index(N, k, j, wrap(i - 1, N));
+  const double *yp = u + NDOF * site_index(N, k, wrap(j + 1, N), i);
+  const double *ym = u + NDOF * site_index(N, k, wrap(j - 1, N), i);
+  const double *zp = u + NDOF * site_index(N, wrap(k + 1, N), j, i);
+  const double *zm = u + NDOF * site_index(N, wrap(k - 1, N), j, i);
+  double hphi_o = 0., hphi_n = 0., s_o = 0., s_n = 0.;
+  double phi_o[4], phi_n[4];
+  for (int l = 0; l < 4; l++) {
+    double heff = (xp[l] + xm[l]) + (yp[l] + ym[l]) + (zp[l] + zm[l]);
+    phi_o[l] = p[l];
+    phi_n[l] = phi_o[l] + rdtg * orc_variance1(g);
+    s_o += phi_o[l] * phi_o[l];
+    s_n += phi_n[l] * phi_n[l];
+    hphi_o += heff * phi_o[l];
+    hphi_n += heff * phi_n[l];
+  }
+  hphi_o += (H * phi_o[0]);
+  hphi_n += (H * phi_n[0]);
+  double dS = -(hphi_n - hphi_o) + Lambda2 * (s_n - s_o) +
+              lambda / 4. * (s_n * s_n - s_o * s_o);
+  if (dS < 0) {
+    for (int l = 0; l < 4; l++) p[l] = store(c, phi_n[l]);
+    mon_down(mon, dS);
+    return;
+  }
+  double r = orc_uniform(g);
+  if (r < exp(-dS)) {
+    for (int l = 0; l < 4; l++) p[l] = store(c, phi_n[l]);
+    mon_up_yes(mon, dS);
+  } else {
+    mon_up_no(mon);
+  }
+}
+
+static void mon_merge(orc_monitor *into, const orc_monitor *m) {
+  into->down_counts += m->down_counts;
+  into->up_counts += m->up_counts;
+  into->up_yes += m->up_yes;
+  into->up_no += m->up_no;
+  into->down_dS += m->down_dS;
+  into->up_dS += m->up_dS;
+}
+
 /* One parity of EulerLangevinHB::step, ModelG/Stepper.cxx:309-386 */
 void orc_hb_phi_halfsweep(orc_ctx *c, double *u, double dt, int ieo) {
   const int N = c->N;
@@ -467,47 +535,25 @@ void orc_hb_phi_halfsweep(orc_ctx *c, double *u, double dt, int ieo) {
   const double rdtg = sqrt(2. * dt * c->gamma);
   uint64_t skey = 0;
   if (c->rng_mode == ORC_RNG_PER_SITE) skey = orc_stream_key(c->base_key, c->sweep_ctr++);
+  if (c->rng_mode == ORC_RNG_PER_SITE && g_threads > 1) {
+#pragma omp parallel num_threads(g_threads)
+    {
+      orc_monitor m = {0};
+#pragma omp for schedule(static)
+      for (int k = 0; k < N; k++)
+        for (int j = 0; j < N; j++)
+          for (int i = 0; i < N; i++)
+            if ((k + j + i) % 2 == ieo) hb_site(c, u, k, j, i, skey, Lambda2, lambda, H, rdtg, &m);
+#pragma omp critical
+      mon_merge(&c->mon_phi, &m);
+    }
+    return;
+  }
   for (int k = 0; k < N; k++)
     for (int j = 0; j < N; j++)
-      for (int i = 0; i < N; i++) {
-        if ((k + j + i) % 2 != ieo) continue;
-        draw_src d;
-        orc_xoro *g = site_engine(c, &d, skey, site_index(N, k, j, i));
-        double *p = u + NDOF * site_index(N, k, j, i);
-        const double *xp = u + NDOF * site_index(N, k, j, wrap(i + 1, N));
-        const double *xm = u + NDOF * site_index(N, k, j, wrap(i - 1, N));
-        const double *yp = u + NDOF * site_index(N, k, wrap(j + 1, N), i);
-        const double *ym = u + NDOF * site_index(N, k, wrap(j - 1, N), i);
-        const double *zp = u + NDOF * site_index(N, wrap(k + 1, N), j, i);
-        const double *zm = u + NDOF * site_index(N, wrap(k - 1, N), j, i);
-        double hphi_o = 0., hphi_n = 0., s_o = 0., s_n = 0.;
-        double phi_o[4], phi_n[4];
-        for (int l = 0; l < 4; l++) {
-          double heff = (xp[l] + xm[l]) + (yp[l] + ym[l]) + (zp[l] + zm[l]);
-          phi_o[l] = p[l];
-          phi_n[l] = phi_o[l] + rdtg * orc_variance1(g);
-          s_o += phi_o[l] * phi_o[l];
-          s_n += phi_n[l] * phi_n[l];
-          hphi_o += heff * phi_o[l];
-          hphi_n += heff * phi_n[l];
-        }
-        hphi_o += (H * phi_o[0]);
-        hphi_n += (H * phi_n[0]);
-        double dS = -(hphi_n - hphi_o) + Lambda2 * (s_n - s_o) +
-                    lambda / 4. * (s_n * s_n - s_o * s_o);
-        if (dS < 0) {
-          for (int l = 0; l < 4; l++) p[l] = store(c, phi_n[l]);
-          mon_down(&c->mon_phi, dS);
-          continue;
-        }
-        double r = orc_uniform(g);
-        if (r < exp(-dS)) {
-          for (int l = 0; l < 4; l++) p[l] = store(c, phi_n[l]);
-          mon_up_yes(&c->mon_phi, dS);
-        } else {
-          mon_up_no(&c->mon_phi);
-        }
-      }
+      for (int i = 0; i < N; i++)
+        if ((k + j + i) % 2 == ieo)
+          hb_site(c, u, k, j, i, skey, Lambda2, lambda, H, rdtg, &c->mon_phi);
 }
 
 /* EulerLangevinHB::step, ModelG/Stepper.cxx:287-389 */
@@ -538,26 +584,44 @@ double orc_update_charge_pair(double chi, double rms, double nA, double nB,
  * ModelG/Stepper.cxx:474-518 with g_face_cases :408-415.  The pairs
  * (A, A+e_dir) with A of parity eoA are disjoint for even N, so updating in
  * place equals the reference's snapshot + scatter-add: nA + (-q), nB + q. */
+static inline void charge_site(orc_ctx *c, double *u, int k, int j, int i, int dir,
+                               double rms, uint64_t skey, orc_monitor *mon) {
+  const int N = c->N;
+  const int iB = dir == 0, jB = dir == 1, kB = dir == 2;
+  draw_src d;
+  orc_xoro *g = site_engine(c, &d, skey, site_index(N, k, j, i));
+  double *pA = u + NDOF * site_index(N, k, j, i);
+  double *pB = u + NDOF * site_index(N, wrap(k + kB, N), wrap(j + jB, N), wrap(i + iB, N));
+  for (int L = 4; L < NDOF; L++) {
+    const double nA = pA[L], nB = pB[L];
+    double q = orc_update_charge_pair(c->chi, rms, nA, nB, g, mon);
+    pA[L] = store(c, nA + (-q));
+    pB[L] = store(c, nB + q);
+  }
+}
+
 void orc_charge_subsweep(orc_ctx *c, double *u, double rms, int dir, int eoA,
                          uint64_t ctr) {
   const int N = c->N;
-  const int iB = dir == 0, jB = dir == 1, kB = dir == 2;
   const uint64_t skey = orc_stream_key(c->base_key, ctr);
+  if (c->rng_mode == ORC_RNG_PER_SITE && g_threads > 1) {
+#pragma omp parallel num_threads(g_threads)
+    {
+      orc_monitor m = {0};
+#pragma omp for schedule(static)
+      for (int k = 0; k < N; k++)
+        for (int j = 0; j < N; j++)
+          for (int i = 0; i < N; i++)
+            if ((k + j + i) % 2 == eoA) charge_site(c, u, k, j, i, dir, rms, skey, &m);
+#pragma omp critical
+      mon_merge(&c->mon_charge, &m);
+    }
+    return;
+  }
   for (int k = 0; k < N; k++)
     for (int j = 0; j < N; j++)
-      for (int i = 0; i < N; i++) {
-        if ((k + j + i) % 2 != eoA) continue;
-        draw_src d;
-        orc_xoro *g = site_engine(c, &d, skey, site_index(N, k, j, i));
-        double *pA = u + NDOF * site_index(N, k, j, i);
-        double *pB = u + NDOF * site_index(N, wrap(k + kB, N), wrap(j + jB, N), wrap(i + iB, N));
-        for (int L = 4; L < NDOF; L++) {
-          const double nA = pA[L], nB = pB[L];
-          double q = orc_update_charge_pair(c->chi, rms, nA, nB, g, &c->mon_charge);
-          pA[L] = store(c, nA + (-q));
-          pB[L] = store(c, nB + q);
-        }
-      }
+      for (int i = 0; i < N; i++)
+        if ((k + j + i) % 2 == eoA) charge_site(c, u, k, j, i, dir, rms, skey, &c->mon_charge);
 }
 
 /* ModelGChargeHB::step, ModelG/Stepper.cxx:449-522 */

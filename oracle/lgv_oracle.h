@@ -90,6 +90,9 @@ typedef struct {
 } orc_ctx;
 
 void orc_ctx_init(orc_ctx *c);
+/* OpenMP threads for the per-site-stream mode (1 = the serial loops). */
+void orc_set_threads(int n);
+int orc_get_threads(void);
 double orc_mass(const orc_ctx *c);
 double orc_energy(const orc_ctx *c, const double *u);
 void orc_rotate_phi(const orc_ctx *c, double *u, double tau);

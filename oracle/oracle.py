@@ -90,6 +90,8 @@ def lib():
         for f in ("orc_mix64",):
             getattr(L, f).argtypes = [C.c_uint64]
             getattr(L, f).restype = C.c_uint64
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_get_threads.restype = C.c_int
         L.orc_base_key.argtypes = [C.c_uint32]
         L.orc_base_key.restype = C.c_uint64
         L.orc_stream_key.argtypes = [C.c_uint64, C.c_uint64]
@@ -390,3 +392,8 @@ def em_adjoint(p, phi0, nsteps, window):
     lib().orc_em_adjoint(C.byref(p), np.ascontiguousarray(phi0).reshape(-1), nsteps,
                          window, g.reshape(-1), gp)
     return g, gp
+
+
+def set_threads(n):
+    """OpenMP threads for the per-site-stream mode's loops (1 = serial)."""
+    lib().orc_set_threads(int(n))

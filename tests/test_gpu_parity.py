@@ -225,3 +225,16 @@ def test_ideal_tile_does_not_change_results(L, orc, N):
             ref = got
         else:
             np.testing.assert_array_equal(got, ref)
+
+
+def test_ideal_tile_rejects_a_ring_larger_than_a_block(L):
+    """At N >= 128 a 64-half-column tile is 4 rows high: its halo ring has
+    2*4 + 4*64 = 264 sites, more than the 256 threads that compute it."""
+    log("tile ring check")
+    from langevina_amd.autotune import set_ideal_tile
+    ctx = L.Context(L.model_params(128, seed=3, **PAR))
+    with pytest.raises(L.LgvError, match="halo ring"):
+        set_ideal_tile(ctx, 0, 64, 16)
+    set_ideal_tile(ctx, 0, 32, 16)  # the context stays usable
+    ctx.step(0.04)
+    ctx.close()

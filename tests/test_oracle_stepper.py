@@ -201,3 +201,29 @@ def test_fp32_mode_stores_float_values(orc):
     for _ in range(2):
         lat.step(0.04)
     assert np.array_equal(lat.u, lat.u.astype(np.float32).astype(np.float64))
+
+
+def test_threaded_per_site_mode_matches_serial(orc):
+    """The OpenMP loops of the CPU baseline change only summation orders:
+    fields and decision counts equal the serial per-site run."""
+    runs = []
+    for nt in (1, 4):
+        orc.set_threads(nt)
+        try:
+            lat = orc.Lattice(12, seed=5, rng_mode=orc.RNG_PER_SITE)
+            lat.init_gaussian_charges()
+            for _ in range(2):
+                lat.step(0.04)
+            runs.append((lat.u.copy(), lat.monitors()))
+        finally:
+            orc.set_threads(1)
+    (u1, m1), (u4, m4) = runs
+    assert np.array_equal(u1, u4)
+    for k in ("phi", "charge", "ideal"):
+        for f in ("down_counts", "up_counts", "up_yes", "up_no"):
+            assert m1[k][f] == m4[k][f]
+        # the ideal-step dS is a difference of two volume sums of O(1e3):
+        # a different summation order moves it by ~1e-12 absolute
+        tol = dict(rel=0, abs=1e-9) if k == "ideal" else dict(rel=1e-12, abs=1e-12)
+        for f in ("down_dS", "up_dS"):
+            assert m4[k][f] == pytest.approx(m1[k][f], **tol)
