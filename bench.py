@@ -279,6 +279,11 @@ def main():
     # synthetic start: phi = 0 (ModelA::initialize), Gaussian charges with the
     # zero mode removed (ModelA::initialize_gaussian_charges)
     ctx.init_gaussian_charges()
+    tuned = {}
+    if not args.no_autotune:
+        # ideal-kernel tile for this lattice size; the tuning steps are warmup
+        from langevina_amd.autotune import autotune_ideal
+        tuned = autotune_ideal(ctx, DT)
     if args.graph:
         ctx.use_graph(True)
     ctx.run(DT, args.warmup)
@@ -330,7 +335,8 @@ def main():
         "config": {"workload": f"ModelG {N}^3 PV2HBSplit23 "
                                f"{'FP64' if prec == L.FP64 else 'FP32 fields'}",
                    "N": N, "dt": DT, "params": PARAMS, "parallelism": "1 GPU",
-                   "graph": bool(args.graph)},
+                   "graph": bool(args.graph),
+                   "ideal_tile": list(tuned[N][:2]) if tuned else "default (32, 16)"},
         "step_roofline_frac": value * ALG_BYTES_PER_SITE_UPDATE * esz / 8 / HBM_PEAK,
         "step_schedule_bytes_frac": value * SCHEDULE_BYTES_PER_SITE_UPDATE * esz / 8 / HBM_PEAK,
         "roofline": {"bound": "hbm", "kernel": KERNEL[dom],

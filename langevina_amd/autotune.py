@@ -11,7 +11,7 @@ import os
 from .lgv import _check, lib
 
 CAPS = (8, 16, 32, 64)
-TZS = (4, 8, 16, 32)
+TZS = (4, 8, 16, 32, 64)
 
 
 def set_ideal_tile(ctx, replica, cap, tz):

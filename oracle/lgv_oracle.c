@@ -556,6 +556,51 @@ void orc_hb_phi_halfsweep(orc_ctx *c, double *u, double dt, int ieo) {
           hb_site(c, u, k, j, i, skey, Lambda2, lambda, H, rdtg, &c->mon_phi);
 }
 
+/* A single site of the phi sweep in the fixed field heff = (h, 0, 0, 0):
+ * n Metropolis updates of hb_site's arithmetic (Stepper.cxx:330-383) after
+ * 10^4 discarded ones, drawing from one engine seeded with seed
+ * (per_site = 0, the reference's layout) or from a fresh per-site stream
+ * keyed by (update counter, site 5) as in the per-site mode.  Returns the
+ * chain means of phi_0 and phi^2 (a check of the sampling against the
+ * exact one-site distribution). */
+void orc_one_site_chain(int per_site, long n, double h, double mass, double lambda, double H,
+                        double dt, uint32_t seed, double out[2]) {
+  const double Lambda2 = 0.5 * (2. * 3. + mass), rdtg = sqrt(2. * dt);
+  double phi[4] = {0., 0., 0., 0.};
+  orc_xoro glob;
+  orc_xoro_seed(&glob, seed);
+  const uint64_t base = orc_base_key(seed);
+  double s0 = 0., s2 = 0.;
+  const long burn = 10000;
+  for (long t = 0; t < n + burn; t++) {
+    orc_xoro loc, *g = &glob;
+    if (per_site) {
+      orc_xoro_seed(&loc, orc_site_key(orc_stream_key(base, (uint64_t)t), 5));
+      g = &loc;
+    }
+    double hphi_o = 0., hphi_n = 0., so = 0., sn = 0., pn[4];
+    for (int l = 0; l < 4; l++) {
+      const double heff = l == 0 ? h : 0.;
+      pn[l] = phi[l] + rdtg * orc_variance1(g);
+      so += phi[l] * phi[l];
+      sn += pn[l] * pn[l];
+      hphi_o += heff * phi[l];
+      hphi_n += heff * pn[l];
+    }
+    hphi_o += H * phi[0];
+    hphi_n += H * pn[0];
+    const double dS = -(hphi_n - hphi_o) + Lambda2 * (sn - so) + lambda / 4. * (sn * sn - so * so);
+    if (dS < 0 || orc_uniform(g) < exp(-dS))
+      for (int l = 0; l < 4; l++) phi[l] = pn[l];
+    if (t >= burn) {
+      s0 += phi[0];
+      s2 += phi[0] * phi[0] + phi[1] * phi[1] + phi[2] * phi[2] + phi[3] * phi[3];
+    }
+  }
+  out[0] = s0 / n;
+  out[1] = s2 / n;
+}
+
 /* EulerLangevinHB::step, ModelG/Stepper.cxx:287-389 */
 void orc_hb_phi_step(orc_ctx *c, double *u, double dt) {
   for (int ieo = 0; ieo < 2; ieo++) orc_hb_phi_halfsweep(c, u, dt, ieo);

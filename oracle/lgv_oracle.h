@@ -100,6 +100,8 @@ void orc_ideal_kick(const orc_ctx *c, double *u, double dt);
 void orc_ideal_step(orc_ctx *c, double *u, double dt);
 void orc_hb_phi_halfsweep(orc_ctx *c, double *u, double dt, int ieo);
 void orc_hb_phi_step(orc_ctx *c, double *u, double dt);
+void orc_one_site_chain(int per_site, long n, double h, double mass, double lambda, double H,
+                        double dt, uint32_t seed, double out[2]);
 void orc_charge_subsweep(orc_ctx *c, double *u, double rms, int dir, int eoA,
                          uint64_t ctr);
 void orc_charge_step(orc_ctx *c, double *u, double dt);

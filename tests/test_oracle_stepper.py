@@ -227,3 +227,29 @@ def test_threaded_per_site_mode_matches_serial(orc):
         tol = dict(rel=0, abs=1e-9) if k == "ideal" else dict(rel=1e-12, abs=1e-12)
         for f in ("down_dS", "up_dS"):
             assert m4[k][f] == pytest.approx(m1[k][f], **tol)
+
+
+@pytest.mark.parametrize("per_site", [0, 1])
+def test_one_site_chain_samples_the_exact_distribution(orc, per_site):
+    """One phi site in a fixed field h e_0 (the sweep's arithmetic, Stepper.cxx
+    :330-383) samples exp(-S), S = -(h+H) phi_0 + Lambda2 phi^2 + lambda/4 phi^4:
+    chain means against 4-D quadrature (phi_0 = r cos(theta), measure
+    r^3 sin^2(theta)), for one reference-layout engine and for fresh per-site
+    streams keyed like the per-site mode."""
+    from scipy import integrate
+    mass, lam, H, h, dt = -4.813, 4.0, 0.01, 1.8, 0.05
+    L2 = 0.5 * (6.0 + mass)
+
+    def w(th, r):
+        return r ** 3 * np.sin(th) ** 2 * np.exp((h + H) * r * np.cos(th) - L2 * r * r
+                                                 - lam / 4 * r ** 4)
+    Z = integrate.dblquad(w, 0, 6, 0, np.pi, epsabs=1e-13, epsrel=1e-12)[0]
+    m0 = integrate.dblquad(lambda th, r: w(th, r) * r * np.cos(th), 0, 6, 0, np.pi,
+                           epsabs=1e-13, epsrel=1e-12)[0] / Z
+    m2 = integrate.dblquad(lambda th, r: w(th, r) * r * r, 0, 6, 0, np.pi,
+                           epsabs=1e-13, epsrel=1e-12)[0] / Z
+    out = np.zeros(2)
+    orc.lib().orc_one_site_chain(per_site, 20_000_000, h, mass, lam, H, dt, 4242, out)
+    # 2e7 updates, integrated autocorrelation ~10: standard errors ~5e-4, ~3e-4
+    assert abs(out[0] - m0) < 2.5e-3, (out[0], m0)
+    assert abs(out[1] - m2) < 1.5e-3, (out[1], m2)
