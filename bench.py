@@ -3,6 +3,7 @@
 
     python bench.py [--steps K] [--warmup W] [--size 256] [--precision fp64|fp32]
                     [--graph] [--batched] [--drift] [--no-cpu-baseline] [--calibrate-copy]
+                    [--no-autotune | --ideal-tile CAP TZ]
 
 One "step" is PV2HBSplit::step(dt) (3 ideal substeps, 12 phi Metropolis
 half-sweeps, 6 charge sub-sweeps; here 3 + 6 + 3 kernel launches) applied to
@@ -26,11 +27,16 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 # (k_charge_march, both sub-sweeps of one direction) read 48 + write 48.
 ALG_BYTES = {"ideal": 160, "phi": 64, "charge": 96}
 KERNEL = {"ideal": "k_ideal_fused", "phi": "k_hb_fused", "charge": "k_charge_march"}
+# FP32 fields run the phi step as two in-place half-sweeps (k_hb_phi): read
+# all phi 32 + write one parity 16 per launch
+ALG_BYTES_FP32 = dict(ALG_BYTES, phi=48)
+KERNEL_FP32 = dict(KERNEL, phi="k_hb_phi")
 # Per site-update with one pass per dependency level of the reference
 # (12 phi half-sweeps of 48 B, 6 charge sub-sweeps of 96 B): 3*160 + 12*48 + 6*96.
 ALG_BYTES_PER_SITE_UPDATE = 1632
 # The same for this schedule: 3 ideal, 6 phi and 3 charge launches.
-SCHEDULE_BYTES_PER_SITE_UPDATE = 3 * 160 + 6 * 64 + 3 * 96  # 1152
+SCHEDULE_BYTES_PER_SITE_UPDATE = 3 * 160 + 6 * 64 + 3 * 96  # 1152 (FP64 fields)
+SCHEDULE_BYTES_PER_SITE_UPDATE_FP32 = 3 * 160 + 12 * 48 + 3 * 96  # 1344, halved for FP32
 # HBM bytes per launch per site measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (kernel, precision, N) -> (fetch, write), each scaled by the
 # copy-kernel calibration (bench.py --calibrate-copy); the summary of those
@@ -39,9 +45,17 @@ MEASURED_TRAFFIC = {}
 
 
 
-def traffic(cls, precision, N):
-    """Measured HBM bytes per launch of the class's kernel at N^3, or None."""
-    m = MEASURED_TRAFFIC.get((KERNEL[cls], precision, N))
+def tables(fp32):
+    """(algorithmic bytes per site per launch, kernel name, schedule bytes per
+    site-update) for the precision's kernel schedule."""
+    if fp32:
+        return ALG_BYTES_FP32, KERNEL_FP32, SCHEDULE_BYTES_PER_SITE_UPDATE_FP32
+    return ALG_BYTES, KERNEL, SCHEDULE_BYTES_PER_SITE_UPDATE
+
+
+def traffic(kernel, precision, N):
+    """Measured HBM bytes per launch of the kernel at N^3, or None."""
+    m = MEASURED_TRAFFIC.get((kernel, precision, N))
     return None if m is None else (m[0] + m[1]) * N ** 3
 
 
@@ -169,10 +183,11 @@ def bench_batched(args):
         c1.close()
     seq_value = sites * args.steps / seq_el
     esz = 8 if prec == L.FP64 else 4
+    alg, kern, _ = tables(prec == L.FP32)
     dom = max(("ideal", "phi", "charge"), key=lambda k: tm[k][0])
     ms, launches = tm[dom]
     per_launch_s = ms * 1e-3 / max(launches, 1)
-    bytes_per_launch = ALG_BYTES[dom] * sites * esz // 8
+    bytes_per_launch = alg[dom] * sites * esz // 8
     achieved = bytes_per_launch / per_launch_s
     out = {
         "metric": "site-updates/sec (ModelG 256^3 FP64 vs FP32/batched) and % HBM roofline, 1 GPU",
@@ -188,7 +203,7 @@ def bench_batched(args):
         "one_at_a_time": {"value": seq_value, "ms_per_step": seq_el * 1e3 / args.steps},
         "batched_over_one_at_a_time": value / seq_value,
         "step_roofline_frac": value * ALG_BYTES_PER_SITE_UPDATE * esz / 8 / HBM_PEAK,
-        "roofline": {"bound": "hbm", "kernel": KERNEL[dom],
+        "roofline": {"bound": "hbm", "kernel": kern[dom],
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None},
     }
@@ -249,6 +264,8 @@ def main():
     ap.add_argument("--batched", action="store_true",
                     help="96 mixed-size replicas in one context (use with --precision fp32)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--ideal-tile", type=int, nargs=2, metavar=("CAP", "TZ"),
+                    help="fix the ideal-kernel tile instead of autotuning it")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the copy kernel used to scale HBM counters")
@@ -280,7 +297,11 @@ def main():
     # zero mode removed (ModelA::initialize_gaussian_charges)
     ctx.init_gaussian_charges()
     tuned = {}
-    if not args.no_autotune:
+    if args.ideal_tile:
+        from langevina_amd.autotune import set_ideal_tile
+        set_ideal_tile(ctx, 0, *args.ideal_tile)
+        tuned = {N: tuple(args.ideal_tile)}
+    elif not args.no_autotune:
         # ideal-kernel tile for this lattice size; the tuning steps are warmup
         from langevina_amd.autotune import autotune_ideal
         tuned = autotune_ideal(ctx, DT)
@@ -309,16 +330,17 @@ def main():
     value = sites * args.steps / el
     ms_per_step = el * 1e3 / args.steps
     # dominant kernel class: the one with the largest total time
+    alg, kern, sched = tables(prec == L.FP32)
     dom = max(("ideal", "phi", "charge"), key=lambda k: tm[k][0])
     ms, launches = tm[dom]
     per_launch_s = ms * 1e-3 / max(launches, 1)
-    bytes_per_launch = ALG_BYTES[dom] * sites * esz // 8
+    bytes_per_launch = alg[dom] * sites * esz // 8
     achieved = bytes_per_launch / per_launch_s
-    kernels = {k: {"ms_total": round(v[0], 3), "launches": v[1],
+    kernels = {k: {"kernel": kern[k], "ms_total": round(v[0], 3), "launches": v[1],
                    "avg_us": round(v[0] * 1e3 / max(v[1], 1), 2),
-                   "alg_GBps": round(ALG_BYTES[k] * sites * esz / 8 /
+                   "alg_GBps": round(alg[k] * sites * esz / 8 /
                                      (v[0] * 1e-3 / max(v[1], 1)) / 1e9, 1) if v[1] else None}
-               for k, v in tm.items() if k in ALG_BYTES}
+               for k, v in tm.items() if k in alg}
     out = {
         "metric": "site-updates/sec (ModelG 256^3 FP64 vs FP32/batched) and % HBM roofline, 1 GPU",
         "value": value,
@@ -338,10 +360,10 @@ def main():
                    "graph": bool(args.graph),
                    "ideal_tile": list(tuned[N][:2]) if tuned else "default (32, 16)"},
         "step_roofline_frac": value * ALG_BYTES_PER_SITE_UPDATE * esz / 8 / HBM_PEAK,
-        "step_schedule_bytes_frac": value * SCHEDULE_BYTES_PER_SITE_UPDATE * esz / 8 / HBM_PEAK,
-        "roofline": {"bound": "hbm", "kernel": KERNEL[dom],
+        "step_schedule_bytes_frac": value * sched * esz / 8 / HBM_PEAK,
+        "roofline": {"bound": "hbm", "kernel": kern[dom],
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic(dom, args.precision, N),
+                     "frac": achieved / HBM_PEAK, "traffic": traffic(kern[dom], args.precision, N),
                      "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": per_launch_s * 1e6},
         "kernels": kernels,
         "measurement": {"ms": meas_s * 1e3, "alg_GBps": N ** 3 * 10 * esz / meas_s / 1e9,

@@ -7,6 +7,7 @@
 #include <limits>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <random>
@@ -405,6 +406,8 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
     whalf.push_back(half);
     wfull.push_back(2 * half);
     if ((uint64_t)half * esz > 0xFFFFFFFFull) c->wide = true;
+    // test hook: LGV_FORCE_WIDE=1 runs the 64-bit-offset kernels on any size
+    if (const char *fw = getenv("LGV_FORCE_WIDE")) c->wide = c->wide || atoi(fw) != 0;
     c->maxN = std::max(c->maxN, N);
   }
   HIPCHECK(hipMalloc(&c->dP, c->nrep * sizeof(ReplicaParams)));

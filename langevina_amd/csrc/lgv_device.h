@@ -196,7 +196,11 @@ LGV_HD void rotate_site(double phi[4], const double A[3], const double V[3],
 // y = |v|^2 (truncation below 1e-18), otherwise with sqrt/sincos.  Agrees
 // with rotate_site to a few ulps (the reference's own libm calls are not
 // bit-reproducible on the device anyway).
+// The ideal step's arithmetic is held to 1e-12, not to bits, so these
+// functions let the compiler contract a*b+c into fused multiply-adds (the
+// rest of the library is built with -ffp-contract=off).
 LGV_HD void cos_sinc(double y, double &c, double &s) {
+#pragma clang fp contract(fast)
   if (y < 0.25) {
     c = 1.0 + y * (-1.0 / 2 + y * (1.0 / 24 + y * (-1.0 / 720 + y * (1.0 / 40320 +
         y * (-1.0 / 3628800 + y * (1.0 / 479001600 + y * (-1.0 / 87178291200.0)))))));
@@ -211,7 +215,17 @@ LGV_HD void cos_sinc(double y, double &c, double &s) {
   }
 }
 
+// The quaternion product of quatmul with contraction allowed.
+LGV_HD void quatmul_fma(const double v1[4], const double v2[4], double w[4]) {
+#pragma clang fp contract(fast)
+  w[0] = v1[0] * v2[0] - (v1[1] * v2[1] + v1[2] * v2[2] + v1[3] * v2[3]);
+  w[1] = v1[0] * v2[1] + v2[0] * v1[1] + (v1[2] * v2[3] - v1[3] * v2[2]);
+  w[2] = v1[0] * v2[2] + v2[0] * v1[2] + (v1[3] * v2[1] - v1[1] * v2[3]);
+  w[3] = v1[0] * v2[3] + v2[0] * v1[3] + (v1[1] * v2[2] - v1[2] * v2[1]);
+}
+
 LGV_HD void rotate_site_fast(double phi[4], const double A[3], const double V[3], double h) {
+#pragma clang fp contract(fast)
   double r[3], l[3];
   for (int i = 0; i < 3; i++) {
     r[i] = h * (V[i] - A[i]);
@@ -223,8 +237,8 @@ LGV_HD void rotate_site_fast(double phi[4], const double A[3], const double V[3]
   const double R[4] = {cR, -sR * r[0], -sR * r[1], -sR * r[2]};
   const double L[4] = {cL, -sL * l[0], -sL * l[1], -sL * l[2]};
   double t[4];
-  quatmul(phi, R, t);
-  quatmul(L, t, phi);
+  quatmul_fma(phi, R, t);
+  quatmul_fma(L, t, phi);
 }
 
 // ---------------------------------------------------------------- sums

@@ -103,6 +103,19 @@ __device__ void block_monitor_add(MonSlot *slot, uint64_t cnt, double dSd, doubl
 }
 
 // ------------------------------------------------------------------------
+// The Metropolis test rr < exp(x), x = -dS <= 0, with the decision of the
+// double exp: a single-precision exp (relative error below 1e-5 for
+// x > -80: one-ulp v_exp_f32 on an argument rounded to float) settles it
+// whenever rr lies outside +-1e-3 of it, the double exp decides the rest.
+__device__ __forceinline__ bool below_exp(double rr, double x) {
+  if (x > -80.) {
+    const double e = (double)__expf((float)x);
+    if (rr < e * (1. - 1e-3)) return true;
+    if (rr >= e * (1. + 1e-3)) return false;
+  }
+  return rr < exp(x);
+}
+
 // The Metropolis decision of one phi site (Stepper.cxx:330-370) in the
 // reference's operand order: heff[l] is the neighbour sum of component l,
 // ((x+1 + x-1) + (y+1 + y-1)) + (z+1 + z-1).  Returns the new value (the
@@ -131,7 +144,7 @@ __device__ __forceinline__ void hb_decide(const double phi_o[4], const double he
     if (count) { cnt += mon_pack(1, 0, 0); dSd += dS; }
   } else {
     const double rr = uniform01(g);
-    accept = rr < exp(-dS);
+    accept = below_exp(rr, -dS);
     if (count) {
       if (accept) { cnt += mon_pack(0, 1, 0); dSu += dS; } else { cnt += mon_pack(0, 0, 1); }
     }
@@ -391,7 +404,7 @@ __device__ __forceinline__ void charge_pair(double nA, double nB, double rms, do
     if (count) { cnt += mon_pack(1, 0, 0); dSd += dS; }
   } else {
     const double rr = uniform01(g);
-    if (rr < exp(-dS)) {
+    if (below_exp(rr, -dS)) {
       if (count) { cnt += mon_pack(0, 1, 0); dSu += dS; }
     } else {
       if (count) cnt += mon_pack(0, 0, 1);
@@ -686,6 +699,8 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
       const uint32_t i = ((uint32_t)z * N + y) * NH + xh;
 #pragma unroll
       for (int s = 0; s < 2; s++) {
+        // held to 1e-12 like the rotations: contraction into FMA allowed
+#pragma clang fp contract(fast)
         const int lx = 2 * tx + s + 1, ly = ty + 1;
         double cp[4], xp[4], xm[4], yp[4], ym[4];
 #pragma unroll

@@ -238,3 +238,89 @@ def test_ideal_tile_rejects_a_ring_larger_than_a_block(L):
     set_ideal_tile(ctx, 0, 32, 16)  # the context stays usable
     ctx.step(0.04)
     ctx.close()
+
+
+@pytest.mark.parametrize("variant", ["nodiffuse", "onlydiffuse"])
+def test_step_variants(L, orc, variant):
+    """PV2HBSplit23NoDiffuse / OnlyDiffuse (SuperPions.cxx:209-225)."""
+    log("variant", variant)
+    N = 16
+    kw = {variant: True}
+    ctx = L.Context(L.model_params(N, seed=5, **PAR), **kw)
+    lat = orc.Lattice(N, seed=5, rng_mode=orc.RNG_PER_SITE, **kw, **{
+        "mass0": PAR["mass0"], "lam": PAR["lam"], "H": PAR["H"], "chi": PAR["chi"]})
+    u = np.random.default_rng(4).normal(size=(N, N, N, 10)) * 0.6
+    lat.u[...] = u
+    ctx.set_field(u)
+    for _ in range(2):
+        ctx.step(0.04)
+        lat.step(0.04)
+    assert rel_err(ctx.get_field(), lat.u) <= 1e-12
+    assert ctx.rng_state().sweep_ctr == lat.c.sweep_ctr
+    ctx.close()
+
+
+def test_lattice_spacing_and_moving_mass(L, orc):
+    """LX = 2 NX (h = 2, the kick's 1/h^2 factor, ModelA.h:229) and
+    m^2(t) = mass0 + dmassdt t."""
+    log("spacing")
+    N = 16
+    par = dict(PAR, dmassdt=-0.7)
+    ctx = L.Context(L.model_params(N, LX=2 * N, seed=8, **par))
+    lat = orc.Lattice(N, LX=2 * N, seed=8, rng_mode=orc.RNG_PER_SITE, dmassdt=-0.7, **{
+        "mass0": PAR["mass0"], "lam": PAR["lam"], "H": PAR["H"], "chi": PAR["chi"]})
+    u = np.random.default_rng(6).normal(size=(N, N, N, 10)) * 0.6
+    lat.u[...] = u
+    ctx.set_field(u)
+    ctx.run(0.04, 3)
+    for _ in range(3):
+        lat.step(0.04)
+        lat.t = lat.t + 0.04
+    assert rel_err(ctx.get_field(), lat.u) <= 1e-12
+    ctx.close()
+
+
+@pytest.mark.parametrize("N", [4, 6, 10])
+def test_small_lattices(L, orc, N):
+    """Sizes where every tile wraps onto itself (N = 4, 6) or is ragged (10)."""
+    log("small", N)
+    ctx, lat = make_pair(L, orc, N, seed=13, scale=0.6)
+    for _ in range(2):
+        ctx.step(0.04)
+        lat.step(0.04)
+    assert rel_err(ctx.get_field(), lat.u) <= 1e-12
+    ctx.close()
+
+
+def test_full_step_fp32(L, orc):
+    """FP32 fields: one full step against the oracle's FP32 mode."""
+    log("full step fp32")
+    N = 16
+    ctx, lat = make_pair(L, orc, N, seed=21, precision=1, scale=0.6)
+    ctx.step(0.04)
+    lat.step(0.04)
+    got = ctx.get_field()
+    assert np.array_equal(got, got.astype(np.float32).astype(np.float64))
+    # the ideal substep's FP64 arithmetic differs from glibc's by an ulp, which
+    # can move a stored float by one ulp
+    np.testing.assert_allclose(got, lat.u, rtol=2.5e-7, atol=1e-30)
+    ctx.close()
+
+
+def test_64bit_offset_kernels_match(L, orc, monkeypatch):
+    """The 64-bit-offset instantiation (planes of 4 GiB or more) is forced on a
+    small lattice and must give the same bits as the 32-bit one."""
+    log("wide offsets")
+    N = 16
+    u = np.random.default_rng(17).normal(size=(N, N, N, 10)) * 0.6
+    out = []
+    for wide in ("0", "1"):
+        monkeypatch.setenv("LGV_FORCE_WIDE", wide)
+        for prec in (0, 1):
+            ctx = L.Context(L.model_params(N, seed=3, **PAR), precision=prec)
+            ctx.set_field(u.astype(np.float32).astype(np.float64) if prec else u)
+            ctx.run(0.04, 2)
+            out.append(ctx.get_field())
+            ctx.close()
+    np.testing.assert_array_equal(out[0], out[2])
+    np.testing.assert_array_equal(out[1], out[3])
