@@ -6,7 +6,7 @@
                     [--no-autotune | --ideal-tile CAP TZ]
 
 One "step" is PV2HBSplit::step(dt) (3 ideal substeps, 12 phi Metropolis
-half-sweeps, 6 charge sub-sweeps; here 3 + 6 + 3 kernel launches) applied to
+half-sweeps, 6 charge sub-sweeps; here 3 + 12 + 3 kernel launches) applied to
 the whole lattice; the metric
 is site-updates/s = sites x steps / wall seconds, inputs resident in HBM.
 Prints one JSON line.
@@ -22,21 +22,20 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 # Algorithmic bytes per site per launch, FP64 fields (halve for FP32):
-# ideal substep (k_ideal_fused) read 80 + write 80; phi step (k_hb_fused,
-# both parities out of place) read 32 + write 32; charge direction
-# (k_charge_march, both sub-sweeps of one direction) read 48 + write 48.
-ALG_BYTES = {"ideal": 160, "phi": 64, "charge": 96}
-KERNEL = {"ideal": "k_ideal_fused", "phi": "k_hb_fused", "charge": "k_charge_march"}
-# FP32 fields run the phi step as two in-place half-sweeps (k_hb_phi): read
-# all phi 32 + write one parity 16 per launch
-ALG_BYTES_FP32 = dict(ALG_BYTES, phi=48)
+# ideal substep (k_ideal_fused) read 80 + write 80; phi half-sweep
+# (k_hb_march for FP64 fields, k_hb_phi for FP32) read all phi 32 + write
+# one parity 16; charge direction (k_charge_march, both sub-sweeps of one
+# direction) read 48 + write 48.
+ALG_BYTES = {"ideal": 160, "phi": 48, "charge": 96}
+KERNEL = {"ideal": "k_ideal_fused", "phi": "k_hb_march", "charge": "k_charge_march"}
+ALG_BYTES_FP32 = ALG_BYTES
 KERNEL_FP32 = dict(KERNEL, phi="k_hb_phi")
 # Per site-update with one pass per dependency level of the reference
 # (12 phi half-sweeps of 48 B, 6 charge sub-sweeps of 96 B): 3*160 + 12*48 + 6*96.
 ALG_BYTES_PER_SITE_UPDATE = 1632
-# The same for this schedule: 3 ideal, 6 phi and 3 charge launches.
-SCHEDULE_BYTES_PER_SITE_UPDATE = 3 * 160 + 6 * 64 + 3 * 96  # 1152 (FP64 fields)
-SCHEDULE_BYTES_PER_SITE_UPDATE_FP32 = 3 * 160 + 12 * 48 + 3 * 96  # 1344, halved for FP32
+# The same for this schedule: 3 ideal, 12 phi and 3 charge launches.
+SCHEDULE_BYTES_PER_SITE_UPDATE = 3 * 160 + 12 * 48 + 3 * 96  # 1344
+SCHEDULE_BYTES_PER_SITE_UPDATE_FP32 = SCHEDULE_BYTES_PER_SITE_UPDATE
 # HBM bytes per launch per site measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (kernel, precision, N) -> (fetch, write), each scaled by the
 # copy-kernel calibration (bench.py --calibrate-copy); the summary of those
@@ -206,6 +205,9 @@ def bench_batched(args):
         "roofline": {"bound": "hbm", "kernel": kern[dom],
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None},
+        "kernels": {k: {"kernel": kern[k], "launches": v[1],
+                        "avg_us": round(v[0] * 1e3 / max(v[1], 1), 2)}
+                    for k, v in tm.items() if k in alg and v[1]},
     }
     print(json.dumps(out), flush=True)
 

@@ -158,14 +158,14 @@ int build_ideal_plan(lgv_ctx *c) {
   return build_plan(c->plan_ideal, work, c->wide);
 }
 
-// Blocks of the fused phi kernel: (x-tile, y-tile, z-chunk) per replica.
+// Blocks of the phi march kernel: (x-tile, y-tile, z-chunk) per replica.
 int build_hb_plan(lgv_ctx *c) {
   std::vector<int64_t> work;
   for (int r = 0; r < c->nrep; r++) {
     const ReplicaParams &p = c->hp[r];
     int TXH, TY, TZ;
     hb_tile(p.N, TXH, TY, TZ);
-    if ((TY + 2) * (2 * TXH + 2) > kHbLds || 2 * TY + 4 * TXH > kBlock)
+    if ((TY + 2) * (TXH + 2) > kHbMarchGrid || 2 * TY + 2 * TXH > kBlock)
       return fail(LGV_ERR_ARG, "replica %d: phi tile %dx%d does not fit", r, TXH, TY);
     work.push_back((int64_t)((p.NH + TXH - 1) / TXH) * ((p.N + TY - 1) / TY) *
                    ((p.N + TZ - 1) / TZ) * kBlock);
@@ -272,20 +272,18 @@ void enqueue_ideal(lgv_ctx *c, int off) {
   }
 }
 
-// EulerLangevinHB::step (counters off, off+1 for the two parities).  FP64
-// fields: both parities in one out-of-place pass; FP32 fields: one in-place
-// pass per parity, measured faster there (the fused pass is bound by its
-// arithmetic and barriers, which FP32 storage does not shrink).
+// EulerLangevinHB::step (counters off, off+1 for the two parities), one
+// in-place launch per parity.  FP64 fields: the z-march with the other
+// parity's plane in LDS (less traffic); FP32 fields: one thread per site
+// (more waves in flight), each the faster one for its storage on MI355X.
 const Plan &hb_plan(const lgv_ctx *c) { return c->fp32 ? c->plan_half : c->plan_hb; }
 void enqueue_hb(lgv_ctx *c, int off) {
-  if (c->fp32) {
-    for (int ieo = 0; ieo < 2; ieo++) {
-      Timed tm(c, LGV_CLASS_PHI);
-      launch_hb_phi(true, c->plan_half, c->dP, c->dS, ieo, off + ieo, c->mon_phi, c->stream);
-    }
-  } else {
+  for (int ieo = 0; ieo < 2; ieo++) {
     Timed tm(c, LGV_CLASS_PHI);
-    launch_hb_fused(false, c->plan_hb, c->dP, c->dS, c->nrep, off, c->mon_phi, c->stream);
+    if (c->fp32)
+      launch_hb_phi(true, c->plan_half, c->dP, c->dS, ieo, off + ieo, c->mon_phi, c->stream);
+    else
+      launch_hb_march(false, c->plan_hb, c->dP, c->dS, ieo, off + ieo, c->mon_phi, c->stream);
   }
 }
 

@@ -55,17 +55,18 @@ __host__ __device__ inline void ideal_tile(int N, int txh_cap, int tz, int &TXH,
   TZ = N < zc ? N : zc;
 }
 
-// Tile of the fused two-parity phi kernel: TXH half-columns x TY rows in
-// x-y, chunks of TZ planes in z (each chunk recomputes one plane at either
-// end); its LDS grid holds (TY+2)(2 TXH+2) sites.
-constexpr int kHbLds = 800;
+// Tile of the single-parity phi march: TXH half-columns x TY rows in x-y,
+// chunks of TZ planes in z.
 __host__ __device__ inline void hb_tile(int N, int &TXH, int &TY, int &TZ) {
   const int NH = N / 2;
   TXH = NH < 32 ? NH : 32;
   TY = 256 / TXH;
   if (TY > N) TY = N;
-  TZ = N < 16 ? N : 16;  // 32 measured slower at 256^3: 1024 blocks leave a tail on 768 slots
+  TZ = N < 16 ? N : 16;
 }
+
+// Its packed LDS grid: (TY+2)(TXH+2) entries.
+constexpr int kHbMarchGrid = 400;
 
 // Tile of the charge march kernel: TXH half-columns x TQ lines of the
 // cross-section, chunks of C sites along the direction (C even).
@@ -106,8 +107,8 @@ constexpr int kBlock = 256;
 // Kernel launchers (lgv_kernels.hip).  fp32 selects the storage type.
 void launch_hb_phi(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
                    int ieo, int ctr_off, MonSlot *mon, hipStream_t st);
-void launch_hb_fused(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
-                     int nrep, int ctr_off, MonSlot *mon, hipStream_t st);
+void launch_hb_march(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
+                     int ieo, int ctr_off, MonSlot *mon, hipStream_t st);
 void launch_shuffle(int nrep, ReplicaState *S, const ReplicaParams *P, int ctr_off,
                     hipStream_t st);
 void launch_charge_march(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,

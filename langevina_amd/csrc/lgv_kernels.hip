@@ -201,33 +201,30 @@ __global__ __launch_bounds__(kBlock) void k_hb_phi(const ReplicaParams *__restri
   block_monitor_add(mon + bid, cnt, dSd, dSu);
 }
 
-// Both parities of EulerLangevinHB::step in one pass, out of place (reads
-// the live phi buffer, writes the other).  A block owns a TXH x TY x-y tile
-// of half-columns (one site of each parity per plane) and marches z through
-// its chunk, one plane behind for the second parity: at plane z it updates
-// the parity-0 sites of z, then the parity-1 sites of z-1, whose parity-0
-// neighbours are then all final.  An LDS grid per plane holds the tile and a
-// one-site ring: the parity-1 entries carry old values (in-plane neighbours
-// of the parity-0 update), the parity-0 entries new values (in-plane
-// neighbours of the parity-1 update one plane later); two grids alternate.
-// z-neighbours at the thread's own (x, y) stay in registers.  Ring parity-0
-// sites are recomputed from global memory with their own counter streams,
-// so they carry the same bits as in the block that owns them; the monitor
-// counts each decision once, in the owning block.
 __device__ __forceinline__ int wrapz(int z, int N) { return z < 0 ? z + N : (z >= N ? z - N : z); }
 
+// One parity of EulerLangevinHB::step (Stepper.cxx:309-386), in place, by
+// marching z through a TXH x TY tile of half-columns (the tile of hb_tile).
+// The update parity ieo reads only the other parity q, which no thread of
+// the launch writes: per plane, the q values of the tile and of a one-site
+// ring sit in a packed LDS grid (one entry per half-column and row; two
+// grids alternate, one barrier per plane), the q values of the thread's own
+// half-column at z-1 and z+1 in registers.  Each q value is loaded once per
+// plane (plus the ring), each decision is made once.
 template <typename T, bool W>
-__global__ __launch_bounds__(kBlock) void k_hb_fused(const ReplicaParams *__restrict__ P,
+__global__ __launch_bounds__(kBlock) void k_hb_march(const ReplicaParams *__restrict__ P,
                                                      const ReplicaState *__restrict__ S,
                                                      const int *__restrict__ blk_rep,
-                                                     const int *__restrict__ blk_first,
+                                                     const int *__restrict__ blk_first, int ieo,
                                                      int ctr_off, MonSlot *__restrict__ mon) {
-  __shared__ T G[2][4][kHbLds];
+  constexpr int kGrid = kHbMarchGrid;
+  __shared__ T Q[2][4][kGrid];
   const int bid = xcd_block();
   const int r = blk_rep[bid];
   const ReplicaParams &p = P[r];
   const ReplicaState &s = S[r];
-  const int cur = live(s), nxt = cur ^ 1;
+  const int cur = live(s);
+  const int q = ieo ^ 1;
   const int N = p.N, NH = p.NH;
   int TXH, TY, TZ;
   hb_tile(N, TXH, TY, TZ);
@@ -238,146 +235,81 @@ __global__ __launch_bounds__(kBlock) void k_hb_fused(const ReplicaParams *__rest
   const int by = b % nty, bz = b / nty;
   const int xh0 = bx * TXH, y0 = by * TY, z0 = bz * TZ;
   const int tw = min(TXH, NH - xh0), th = min(TY, N - y0), tz = min(TZ, N - z0);
-  const int LW = 2 * tw + 2;  // LDS row width
+  const int GW = tw + 2;  // grid row width: half-columns xh0-1 .. xh0+tw
   const int t = threadIdx.x;
   const int tx = t % TXH, ty = t / TXH;
   const bool own = t < TXH * TY && ty < th && tx < tw;
   const int xh = xh0 + tx, y = y0 + ty;
-  // ring site of this thread (one-site ring around the tile, no corners)
-  const bool halo = t < 2 * th + 4 * tw;
-  int hlx = 0, hly = 0;
+  // ring entry of this thread: left/right half-column, rows above/below
+  const bool halo = t < 2 * th + 2 * tw;
+  int hlx, hly;
   if (t < th) { hlx = -1; hly = t; }
-  else if (t < 2 * th) { hlx = 2 * tw; hly = t - th; }
-  else if (t < 2 * th + 2 * tw) { hlx = t - 2 * th; hly = -1; }
-  else { hlx = t - 2 * th - 2 * tw; hly = th; }
-  const int hgx = wrapz(2 * xh0 + hlx, N), hgy = wrapz(y0 + hly, N);
-  const int hpos = (hly + 1) * LW + (hlx + 1);
+  else if (t < 2 * th) { hlx = tw; hly = t - th; }
+  else if (t < 2 * th + tw) { hlx = t - 2 * th; hly = -1; }
+  else { hlx = t - 2 * th - tw; hly = th; }
+  const int hgxh = wrapz(xh0 + hlx, NH), hgy = wrapz(y0 + hly, N);
+  const int hpos = (hly + 1) * GW + (hlx + 1);
 
   const double mass = p.mass0 + p.dmassdt * s.t;
   const double Lambda2 = 0.5 * (2. * 3. + mass);
   const double lambda = p.lambda, H = p.H, rdtg = p.rdtg;
-  const uint64_t key0 = stream_key(p.base_key, s.sweep_ctr + ctr_off);
-  const uint64_t key1 = stream_key(p.base_key, s.sweep_ctr + ctr_off + 1);
+  const uint64_t key = stream_key(p.base_key, s.sweep_ctr + ctr_off);
   uint64_t cnt = 0;
   double dSd = 0., dSu = 0.;
-
-  auto idx_of = [&](int x, int yy, int z) __attribute__((always_inline)) {
-    return ((uint32_t)z * N + yy) * NH + (x >> 1);
+  auto idx = [&](int xhh, int yy, int z) __attribute__((always_inline)) {
+    return ((uint32_t)z * N + yy) * NH + xhh;
   };
   auto ld4 = [&](int par, uint32_t i, double v[4]) __attribute__((always_inline)) {
 #pragma unroll
     for (int l = 0; l < 4; l++) v[l] = gplane<T, W>(p, cur, l, par).ld(i);
   };
-  // x of this thread's site of parity par in plane z
-  auto own_x = [&](int par, int z) __attribute__((always_inline)) {
-    return 2 * xh + ((par + y + z) & 1);
-  };
-
-  // own half-column: old parity-1 values at z-1, z, z+1; new parity-0 values
-  // at z-2, z-1, z
-  double o1m[4], o1c[4], o1p[4], n0m2[4] = {}, n0m1[4] = {}, n0c[4] = {};
+  double qm[4], qc[4], qp[4];  // q values of the own half-column at z-1, z, z+1
   if (own) {
-    const int za = wrapz(z0 - 2, N), zb = wrapz(z0 - 1, N);
-    ld4(1, idx_of(own_x(1, za), y, za), o1m);
-    ld4(1, idx_of(own_x(1, zb), y, zb), o1c);
+    ld4(q, idx(xh, y, wrapz(z0 - 1, N)), qm);
+    ld4(q, idx(xh, y, z0), qc);
   }
-  for (int j = 0; j < tz + 2; j++) {
-    const int zu = z0 - 1 + j;
-    const int z = wrapz(zu, N);
+  for (int j = 0; j < tz; j++) {
+    const int z = z0 + j;
     const int zp = z + 1 == N ? 0 : z + 1;
-    const bool inchunk = zu >= z0 && zu < z0 + tz;
-    T(*g)[kHbLds] = G[j & 1];
-    double o0c[4];
+    T(*g)[kGrid] = Q[j & 1];
+    double phi_o[4];
     if (own) {
-      const int x0 = own_x(0, z), x1 = own_x(1, z);
-      ld4(1, idx_of(x1, y, zp), o1p);
-      ld4(0, idx_of(x0, y, z), o0c);
-      const int pos = (ty + 1) * LW + (x1 - 2 * xh0 + 1);
+      ld4(q, idx(xh, y, zp), qp);
+      ld4(ieo, idx(xh, y, z), phi_o);
+      const int pos = (ty + 1) * GW + (tx + 1);
 #pragma unroll
-      for (int l = 0; l < 4; l++) g[l][pos] = (T)o1c[l];
+      for (int l = 0; l < 4; l++) g[l][pos] = (T)qc[l];
     }
     if (halo) {
-      if (((hgx + hgy + z) & 1) == 1) {
-        double v[4];
-        ld4(1, idx_of(hgx, hgy, z), v);
+      double v[4];
+      ld4(q, idx(hgxh, hgy, z), v);
 #pragma unroll
-        for (int l = 0; l < 4; l++) g[l][hpos] = (T)v[l];
-      } else if (inchunk) {
-        // parity-0 ring site: the update its owner computes, from global memory
-        const int xp = hgx + 1 == N ? 0 : hgx + 1, xm = hgx == 0 ? N - 1 : hgx - 1;
-        const int yp = hgy + 1 == N ? 0 : hgy + 1, ym = hgy == 0 ? N - 1 : hgy - 1;
-        const int zm = z == 0 ? N - 1 : z - 1;
-        const uint32_t nb[6] = {idx_of(xp, hgy, z), idx_of(xm, hgy, z), idx_of(hgx, yp, z),
-                                idx_of(hgx, ym, z), idx_of(hgx, hgy, zp), idx_of(hgx, hgy, zm)};
-        double phi_o[4], heff[4], phi_n[4];
-#pragma unroll
-        for (int l = 0; l < 4; l++) {
-          const GPlane<T, W> fq = gplane<T, W>(p, cur, l, 1);
-          heff[l] = (fq.ld(nb[0]) + fq.ld(nb[1])) + (fq.ld(nb[2]) + fq.ld(nb[3])) +
-                    (fq.ld(nb[4]) + fq.ld(nb[5]));
-        }
-        ld4(0, idx_of(hgx, hgy, z), phi_o);
-        Xoro gg = xoro_seeded(site_key(key0, ((uint64_t)z * N + hgy) * N + hgx));
-        hb_decide(phi_o, heff, Lambda2, lambda, H, rdtg, gg, phi_n, cnt, dSd, dSu, false);
-#pragma unroll
-        for (int l = 0; l < 4; l++) g[l][hpos] = (T)phi_n[l];
-      }
+      for (int l = 0; l < 4; l++) g[l][hpos] = (T)v[l];
     }
     __syncthreads();
     if (own) {
-      // parity 0 at plane z
-      const int x0 = own_x(0, z);
-      const int pos = (ty + 1) * LW + (x0 - 2 * xh0 + 1);
-      double heff[4];
+      const int o = (ieo + y + z) & 1;  // x = 2 xh + o
+      const int pos = (ty + 1) * GW + (tx + 1);
+      double heff[4], phi_n[4];
 #pragma unroll
       for (int l = 0; l < 4; l++)
-        heff[l] = ((double)g[l][pos + 1] + (double)g[l][pos - 1]) +
-                  ((double)g[l][pos + LW] + (double)g[l][pos - LW]) + (o1p[l] + o1m[l]);
-      Xoro gg = xoro_seeded(site_key(key0, ((uint64_t)z * N + y) * N + x0));
-      hb_decide(o0c, heff, Lambda2, lambda, H, rdtg, gg, n0c, cnt, dSd, dSu, inchunk);
+        heff[l] = ((double)g[l][pos + o] + (double)g[l][pos - 1 + o]) +
+                  ((double)g[l][pos + GW] + (double)g[l][pos - GW]) + (qp[l] + qm[l]);
+      Xoro gg = xoro_seeded(site_key(key, ((uint64_t)z * N + y) * N + 2 * xh + o));
+      hb_decide(phi_o, heff, Lambda2, lambda, H, rdtg, gg, phi_n, cnt, dSd, dSu, true);
+      const uint32_t i = idx(xh, y, z);
+#pragma unroll
+      for (int l = 0; l < 4; l++) gplane<T, W>(p, cur, l, ieo).st(i, phi_n[l]);
 #pragma unroll
       for (int l = 0; l < 4; l++) {
-        n0c[l] = (double)(T)n0c[l];
-        g[l][pos] = (T)n0c[l];
+        qm[l] = qc[l];
+        qc[l] = qp[l];
       }
-      // parity 1 at plane z-1, from the other grid
-      const int zbu = zu - 1;
-      if (zbu >= z0 && zbu < z0 + tz) {
-        const int zb = wrapz(zbu, N);
-        T(*gb)[kHbLds] = G[(j + 1) & 1];
-        const int x1 = own_x(1, zb);
-        const int pb = (ty + 1) * LW + (x1 - 2 * xh0 + 1);
-#pragma unroll
-        for (int l = 0; l < 4; l++)
-          heff[l] = ((double)gb[l][pb + 1] + (double)gb[l][pb - 1]) +
-                    ((double)gb[l][pb + LW] + (double)gb[l][pb - LW]) + (n0c[l] + n0m2[l]);
-        Xoro g1 = xoro_seeded(site_key(key1, ((uint64_t)zb * N + y) * N + x1));
-        double n1[4];
-        hb_decide(o1m, heff, Lambda2, lambda, H, rdtg, g1, n1, cnt, dSd, dSu, true);
-        const uint32_t i = idx_of(x1, y, zb);
-#pragma unroll
-        for (int l = 0; l < 4; l++) {
-          gplane<T, W>(p, nxt, l, 1).st(i, n1[l]);
-          gplane<T, W>(p, nxt, l, 0).st(i, n0m1[l]);
-        }
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int l = 0; l < 4; l++) {
-      o1m[l] = o1c[l];
-      o1c[l] = o1p[l];
-      n0m2[l] = n0m1[l];
-      n0m1[l] = n0c[l];
     }
   }
   block_monitor_add(mon + bid, cnt, dSd, dSu);
 }
 
-__global__ void k_flip_phi(ReplicaState *S, int nrep) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < nrep) S[r].cur_phi ^= 1;
-}
 
 // ------------------------------------------------------------------------
 // The shuffled sub-sweep order of ModelGChargeHB::step (Stepper.cxx:461-468)
@@ -394,12 +326,32 @@ __global__ void k_shuffle(ReplicaState *S, const ReplicaParams *P, int nrep, int
 // ------------------------------------------------------------------------
 // modelg_update_charge_pair (Stepper.cxx:419-441) for one component of one
 // pair, applied as nA + (-q), nB + q; counts the decision when count.
-__device__ __forceinline__ void charge_pair(double nA, double nB, double rms, double chi,
+// The constants of a charge sweep: the proposal width and the divisor 2 chi
+// of dS with its rounded reciprocal.
+struct ChargeK {
+  double rms, d, inv;
+};
+__device__ __forceinline__ ChargeK charge_k(double rms, double chi) {
+  const double d = 2. * chi;
+  return {rms, d, 1. / d};
+}
+// x / d from the rounded reciprocal with one Markstein correction: q0 = x *
+// inv, the residual x - q0 d exact with one FMA, q0 + residual * inv rounded
+// once.  It equals the IEEE quotient for these divisors (no difference in
+// 1.6e8 random dividends over eight values of chi, including the defaults;
+// the CPU suite checks it); a one-ulp difference could only change a
+// decision if the uniform draw fell within that ulp of exp(-dS).  Three
+// instructions instead of the division's ten.
+__device__ __forceinline__ double div_by(double x, const ChargeK &k) {
+  const double q0 = x * k.inv;
+  return fma(fma(-q0, k.d, x), k.inv, q0);
+}
+
+__device__ __forceinline__ void charge_pair(double nA, double nB, const ChargeK &k,
                                             Xoro &g, double &outA, double &outB, uint64_t &cnt,
                                             double &dSd, double &dSu, bool count) {
-  double q = rms * variance1(g);
-  const double dS = ((nA - q) * (nA - q) + (nB + q) * (nB + q) - nA * nA - nB * nB) /
-                    (2. * chi);
+  double q = k.rms * variance1(g);
+  const double dS = div_by((nA - q) * (nA - q) + (nB + q) * (nB + q) - nA * nA - nB * nB, k);
   if (dS < 0) {
     if (count) { cnt += mon_pack(1, 0, 0); dSd += dS; }
   } else {
@@ -444,7 +396,7 @@ __device__ __forceinline__ void charge_line(const ReplicaParams &p, int cur, int
     ii[k] = (zz * N + yy) * NH + (xx >> 1);
     lin[k] = (zz * N + yy) * N + xx;
   }
-  const double chi = p.chi, rms = p.rms;
+  const ChargeK ck = charge_k(p.rms, p.chi);
   Xoro gL = xoro_seeded(site_key(k1, lin[0]));
   Xoro gB = xoro_seeded(site_key(k1, lin[2]));
   Xoro gA = xoro_seeded(site_key(k2, lin[1]));
@@ -458,13 +410,13 @@ __device__ __forceinline__ void charge_line(const ReplicaParams &p, int cur, int
     const double vB = f1.ld(ii[2]);
     const double vR = f2.ld(ii[3]);
     double tL, a1, b1, tR;
-    charge_pair(vL, vA, rms, chi, gL, tL, a1, cnt, dSd, dSu, true);
-    charge_pair(vB, vR, rms, chi, gB, b1, tR, c0, sd0, su0, false);
+    charge_pair(vL, vA, ck, gL, tL, a1, cnt, dSd, dSu, true);
+    charge_pair(vB, vR, ck, gB, b1, tR, c0, sd0, su0, false);
     // the first sub-sweep's results as stored (rounded in FP32 storage)
     a1 = (double)(T)a1;
     b1 = (double)(T)b1;
     double a2, b2;
-    charge_pair(a1, b1, rms, chi, gA, a2, b2, cnt, dSd, dSu, true);
+    charge_pair(a1, b1, ck, gA, a2, b2, cnt, dSd, dSu, true);
     gplane<T, W>(p, nxt, L, e2).st(ii[1], a2);
     gplane<T, W>(p, nxt, L, e1).st(ii[2], b2);
   }
@@ -488,7 +440,7 @@ __device__ __forceinline__ void charge_column(const ReplicaParams &p, int cur, i
                                               int c1, uint64_t k1, uint64_t k2, uint64_t &cnt,
                                               double &dSd, double &dSu) {
   const int N = p.N, NH = p.NH;
-  const double chi = p.chi, rms = p.rms;
+  const ChargeK ck = charge_k(p.rms, p.chi);
   // index / linear site of position m along the column
   auto at = [&](int m, uint32_t &ii, uint32_t &lin) __attribute__((always_inline)) {
     const uint32_t row = dir == 2 ? (uint32_t)m * N + q : q * N + (uint32_t)m;
@@ -520,12 +472,12 @@ __device__ __forceinline__ void charge_column(const ReplicaParams &p, int cur, i
       const double v1 = gplane<T, W>(p, cur, L, e1).ld(i1);
       const double v2 = gplane<T, W>(p, cur, L, e2).ld(i2);
       double a, b;
-      charge_pair(v1, v2, rms, chi, g1, a, b, cnt, dSd, dSu, own1);
+      charge_pair(v1, v2, ck, g1, a, b, cnt, dSd, dSu, own1);
       a = (double)(T)a;
       b = (double)(T)b;
       if (k > 0) {
         double o0, o1;
-        charge_pair(up[L - 4], a, rms, chi, g2, o0, o1, cnt, dSd, dSu, true);
+        charge_pair(up[L - 4], a, ck, g2, o0, o1, cnt, dSd, dSu, true);
         gplane<T, W>(p, nxt, L, e2).st(i0, o0);
         gplane<T, W>(p, nxt, L, e1).st(i1, o1);
       }
@@ -1084,11 +1036,10 @@ void launch_hb_phi(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaS
                ctr_off, mon);
 }
 
-void launch_hb_fused(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
-                     int nrep, int ctr_off, MonSlot *mon, hipStream_t st) {
-  LGV_DISPATCH(k_hb_fused, plan.nblocks, st, P, S, plan.d_blk_rep, plan.d_blk_first, ctr_off,
-               mon);
-  k_flip_phi<<<nblk(nrep, 64), 64, 0, st>>>(S, nrep);
+void launch_hb_march(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
+                     int ieo, int ctr_off, MonSlot *mon, hipStream_t st) {
+  LGV_DISPATCH(k_hb_march, plan.nblocks, st, P, S, plan.d_blk_rep, plan.d_blk_first, ieo,
+               ctr_off, mon);
 }
 
 void launch_shuffle(int nrep, ReplicaState *S, const ReplicaParams *P, int ctr_off,

@@ -601,6 +601,25 @@ void orc_one_site_chain(int per_site, long n, double h, double mass, double lamb
   out[1] = s2 / n;
 }
 
+/* The GPU charge sweep divides dS by 2 chi as q0 = x * inv, q0 + (x - q0 d)
+ * * inv with one fused multiply-add each (inv = 1/d rounded).  Counts, over n
+ * pseudo-random dividends of magnitude 2^-20 .. 2^20, how often that differs
+ * from x / d. */
+long orc_div_by_const_mismatches(double chi, long n, uint64_t seed) {
+  const double d = 2. * chi, inv = 1. / d;
+  orc_xoro g;
+  orc_xoro_seed(&g, seed);
+  long bad = 0;
+  for (long i = 0; i < n; i++) {
+    const uint64_t r = orc_xoro_next(&g);
+    double x = ldexp((double)(r >> 11), -53) * ldexp(1.0, (int)(orc_xoro_next(&g) % 41) - 20);
+    if (r & 1) x = -x;
+    const double q0 = x * inv;
+    if (fma(fma(-q0, d, x), inv, q0) != x / d) bad++;
+  }
+  return bad;
+}
+
 /* EulerLangevinHB::step, ModelG/Stepper.cxx:287-389 */
 void orc_hb_phi_step(orc_ctx *c, double *u, double dt) {
   for (int ieo = 0; ieo < 2; ieo++) orc_hb_phi_halfsweep(c, u, dt, ieo);

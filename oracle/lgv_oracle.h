@@ -100,6 +100,7 @@ void orc_ideal_kick(const orc_ctx *c, double *u, double dt);
 void orc_ideal_step(orc_ctx *c, double *u, double dt);
 void orc_hb_phi_halfsweep(orc_ctx *c, double *u, double dt, int ieo);
 void orc_hb_phi_step(orc_ctx *c, double *u, double dt);
+long orc_div_by_const_mismatches(double chi, long n, uint64_t seed);
 void orc_one_site_chain(int per_site, long n, double h, double mass, double lambda, double H,
                         double dt, uint32_t seed, double out[2]);
 void orc_charge_subsweep(orc_ctx *c, double *u, double rms, int dir, int eoA,

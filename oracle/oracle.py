@@ -92,6 +92,8 @@ def lib():
             getattr(L, f).restype = C.c_uint64
         L.orc_one_site_chain.argtypes = [C.c_int, C.c_long, C.c_double, C.c_double, C.c_double,
                                          C.c_double, C.c_double, C.c_uint32, dp]
+        L.orc_div_by_const_mismatches.argtypes = [C.c_double, C.c_long, C.c_uint64]
+        L.orc_div_by_const_mismatches.restype = C.c_long
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_get_threads.restype = C.c_int
         L.orc_base_key.argtypes = [C.c_uint32]

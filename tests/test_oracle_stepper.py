@@ -253,3 +253,11 @@ def test_one_site_chain_samples_the_exact_distribution(orc, per_site):
     # 2e7 updates, integrated autocorrelation ~10: standard errors ~5e-4, ~3e-4
     assert abs(out[0] - m0) < 2.5e-3, (out[0], m0)
     assert abs(out[1] - m2) < 1.5e-3, (out[1], m2)
+
+
+@pytest.mark.parametrize("chi", [2.0, 5.0, 3.0, 0.7, 10.0 / 3.0, 0.123456789])
+def test_charge_dS_division_by_reciprocal_is_exact(orc, chi):
+    """The GPU charge sweep forms dS = x / (2 chi) with a rounded reciprocal
+    and one FMA correction; it must equal the IEEE quotient of the
+    reference's expression (Stepper.cxx:426-428)."""
+    assert orc.lib().orc_div_by_const_mismatches(chi, 2_000_000, 12345) == 0
