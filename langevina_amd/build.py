@@ -41,22 +41,34 @@ DRIVER = os.path.join(BIN, "lgv_superpions")
 HDF5_INCLUDE = os.environ.get("LGV_HDF5_INCLUDE", "/opt/conda/include")
 
 
-def build_driver(verbose=False):
-    """The C++ run driver (host code only): HDF5 headers at compile time,
-    libhdf5 dlopen'ed at run time, liblgv_hip.so found through $ORIGIN."""
-    os.makedirs(BIN, exist_ok=True)
-    tmp = DRIVER + ".tmp"
+# offline post-processing tools, named as ModelG/x2k.py looks them up
+# ($MODELGPATH/x2k.exe, $MODELGPATH/x2k_rotated.exe)
+TOOLS = {"x2k.exe": "x2k.cpp", "x2k_rotated.exe": "x2k_rotated.cpp"}
+
+
+def _host_program(out, src, link):
+    tmp = out + ".tmp"
     cmd = ["g++", "-O2", "-std=c++17", "-Wall",
            "-I", os.path.join(ROOT, "include"), "-I", os.path.join(PKG, "host"),
-           "-idirafter", HDF5_INCLUDE,
-           os.path.join(PKG, "host", "superpions.cpp"), "-o", tmp,
-           "-L", PKG, "-llgv_hip", "-Wl,-rpath,$ORIGIN/..", "-ldl"]
+           "-idirafter", HDF5_INCLUDE, os.path.join(PKG, "host", src), "-o", tmp, *link]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"driver build failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, DRIVER)
+        raise RuntimeError(f"build of {src} failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, out)
+
+
+def build_driver(verbose=False):
+    """The C++ run driver and the x2k tools (host code only): HDF5 headers at
+    compile time, libhdf5 dlopen'ed at run time, liblgv_hip.so (driver only)
+    found through $ORIGIN."""
+    os.makedirs(BIN, exist_ok=True)
+    jobs = [(DRIVER, "superpions.cpp", ["-L", PKG, "-llgv_hip", "-Wl,-rpath,$ORIGIN/..", "-ldl"])]
+    jobs += [(os.path.join(BIN, exe), src, ["-ldl"]) for exe, src in TOOLS.items()]
+    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+        list(ex.map(lambda j: _host_program(*j), jobs))
     if verbose:
-        print("built", DRIVER)
+        for out, _, _ in jobs:
+            print("built", out)
     return DRIVER
 
 

@@ -126,76 +126,19 @@ LGV_HD uint64_t site_key(uint64_t skey, uint64_t site) {
 // A half-sweep therefore reads one half-lattice and writes the other
 // densely, and x-neighbours sit in the adjacent or the same column.
 
-struct Geo {
-  int N, NH;       // side and half side
-  int64_t half;    // N^3 / 2
-};
-
 LGV_HD int site_x(int xh, int y, int z, int p) { return 2 * xh + ((p + y + z) & 1); }
 
 // ---------------------------------------------------------------- O(4)
-// ModelG/O4AlgebraHelper.h, expression by expression.
-
-LGV_HD void expmul(const double v[3], double w[4]) {
-  const double abs = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-  if (abs > 1e-20) {
-    double s, c;
-    s = sin(abs);
-    c = cos(abs);
-    w[0] = c;
-    const double a = -(s / abs);
-    w[1] = a * v[0];
-    w[2] = a * v[1];
-    w[3] = a * v[2];
-  } else {
-    w[0] = 1; w[1] = 0.0; w[2] = 0.0; w[3] = 0.0;
-  }
-}
-
-LGV_HD void quatmul(const double v1[4], const double v2[4], double w[4]) {
-  double dot = 0;
-  dot += v1[1] * v2[1];
-  dot += v1[2] * v2[2];
-  dot += v1[3] * v2[3];
-  w[0] = v1[0] * v2[0] - dot;
-  w[1] = v1[2] * v2[3] - v2[2] * v1[3];
-  w[2] = v1[3] * v2[1] - v1[1] * v2[3];
-  w[3] = v1[1] * v2[2] - v2[1] * v1[2];
-  w[1] += v2[0] * v1[1];
-  w[2] += v2[0] * v1[2];
-  w[3] += v2[0] * v1[3];
-  w[1] += v1[0] * v2[1];
-  w[2] += v1[0] * v2[2];
-  w[3] += v1[0] * v2[3];
-}
-
 // IdealPV2::rotatePhi's per-site work (Stepper.cxx:186-198) with
-// O4Rotation (O4AlgebraHelper.h:78-97): mu = -charge/chi*tau, then
-// phi <- L (phi R).
-LGV_HD void rotate_site(double phi[4], const double A[3], const double V[3],
-                        double chi, double tau) {
-  double vmu[3], amu[3];
-  for (int s = 0; s < 3; s++) vmu[s] = -V[s] / chi * tau;
-  for (int s = 0; s < 3; s++) amu[s] = -A[s] / chi * tau;
-  double tmp1[4], tmp2[4], tmp3[4], out[4];
-  for (int i = 0; i < 3; ++i) {
-    tmp1[i] = -0.5 * (vmu[i] - amu[i]);
-    tmp2[i] = 0.5 * (vmu[i] + amu[i]);
-  }
-  expmul(tmp1, tmp3);
-  expmul(tmp2, tmp1);
-  quatmul(phi, tmp3, tmp2);
-  quatmul(tmp1, tmp2, out);
-  phi[0] = out[0]; phi[1] = out[1]; phi[2] = out[2]; phi[3] = out[3];
-}
-
-// The same rotation for the fused ideal kernel.  With k = tau/chi the two
-// generators are tmp1 = -0.5(mu_V - mu_A) = h (V - A) and
-// tmp2 = 0.5(mu_V + mu_A) = -h (V + A), h = k/2.  expmul needs cos|v| and
-// sin|v|/|v|; for |v| < 1/2 both are evaluated as Taylor series in
-// y = |v|^2 (truncation below 1e-18), otherwise with sqrt/sincos.  Agrees
-// with rotate_site to a few ulps (the reference's own libm calls are not
-// bit-reproducible on the device anyway).
+// O4Rotation, expmul and quatmul (O4AlgebraHelper.h:54-97): with
+// mu = -charge/chi * tau, phi <- L (phi R), R = expmul(-(mu_V - mu_A)/2),
+// L = expmul((mu_V + mu_A)/2).
+// With k = tau/chi the two generators are tmp1 = -0.5(mu_V - mu_A) =
+// h (V - A) and tmp2 = 0.5(mu_V + mu_A) = -h (V + A), h = k/2.  expmul needs
+// cos|v| and sin|v|/|v|; for |v| < 1/2 both are evaluated as Taylor series
+// in y = |v|^2 (truncation below 1e-18), otherwise with sqrt/sincos.  Agrees
+// with the reference's expression order to a few ulps (the reference's own
+// libm calls are not bit-reproducible on the device anyway).
 // The ideal step's arithmetic is held to 1e-12, not to bits, so these
 // functions let the compiler contract a*b+c into fused multiply-adds (the
 // rest of the library is built with -ffp-contract=off).

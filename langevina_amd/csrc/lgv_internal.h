@@ -56,10 +56,11 @@ __host__ __device__ inline void ideal_tile(int N, int txh_cap, int tz, int &TXH,
 }
 
 // Tile of the single-parity phi march: TXH half-columns x TY rows in x-y,
-// chunks of TZ planes in z.
+// chunks of TZ planes in z (64 x 4 x 16 from N = 128 on: wide rows keep the
+// x-halo columns a small share of the lines each plane touches).
 __host__ __device__ inline void hb_tile(int N, int &TXH, int &TY, int &TZ) {
   const int NH = N / 2;
-  TXH = NH < 32 ? NH : 32;
+  TXH = NH < 64 ? NH : 64;
   TY = 256 / TXH;
   if (TY > N) TY = N;
   TZ = N < 16 ? N : 16;

@@ -57,8 +57,18 @@ __device__ __forceinline__ GPlane<T, W> gplane(const ReplicaParams &p, int cur, 
 
 // Block index of this workgroup within the launch plan.  (A renumbering that
 // gives each XCD a contiguous eighth of the plan was measured on MI355X and
-// made the sweeps slower; blocks run in plain dispatch order.)
+// made the ideal and charge sweeps slower; they run in plain dispatch order.)
 __device__ __forceinline__ int xcd_block() { return blockIdx.x; }
+
+// The renumbering itself, used by the phi march: the dispatcher deals
+// workgroups round-robin over the 8 XCDs, so workgroup b runs on XCD b % 8;
+// plan block (b % 8) * (G/8) + b / 8 then puts x-adjacent tiles, which share
+// the halo columns' cache lines, into one XCD's L2.  A bijection on the first
+// 8 * (G/8) blocks; the tail keeps its index.
+__device__ __forceinline__ int xcd_contiguous_block() {
+  const int b = blockIdx.x, g8 = (int)gridDim.x >> 3;
+  return b < (g8 << 3) ? (b & 7) * g8 + (b >> 3) : b;
+}
 
 __device__ __forceinline__ int wrapp(int a, int N) { return a == N ? 0 : a; }
 __device__ __forceinline__ int wrapm(int a, int N) { return a < 0 ? N - 1 : a; }
@@ -219,7 +229,7 @@ __global__ __launch_bounds__(kBlock) void k_hb_march(const ReplicaParams *__rest
                                                      int ctr_off, MonSlot *__restrict__ mon) {
   constexpr int kGrid = kHbMarchGrid;
   __shared__ T Q[2][4][kGrid];
-  const int bid = xcd_block();
+  const int bid = xcd_contiguous_block();
   const int r = blk_rep[bid];
   const ReplicaParams &p = P[r];
   const ReplicaState &s = S[r];

@@ -43,6 +43,8 @@ struct H5Api {
   decltype(&H5Dwrite) Dwrite;
   decltype(&H5Dread) Dread;
   decltype(&H5Lexists) Lexists;
+  decltype(&H5Ldelete) Ldelete;
+  decltype(&H5Sget_simple_extent_ndims) Sget_simple_extent_ndims;
   hid_t native_double = -1, dataset_create = -1;
 
   template <typename F>
@@ -86,6 +88,8 @@ struct H5Api {
     sym(Dwrite, "H5Dwrite");
     sym(Dread, "H5Dread");
     sym(Lexists, "H5Lexists");
+    sym(Ldelete, "H5Ldelete");
+    sym(Sget_simple_extent_ndims, "H5Sget_simple_extent_ndims");
     if (open_() < 0) throw std::runtime_error("H5open failed");
     hid_t *nd = reinterpret_cast<hid_t *>(dlsym(lib, "H5T_NATIVE_DOUBLE_g"));
     hid_t *dc = reinterpret_cast<hid_t *>(dlsym(lib, "H5P_CLS_DATASET_CREATE_ID_g"));
@@ -133,6 +137,31 @@ class Ntuple {
     pack(line, 1);
     mem_ = h.Screate_simple(rank + 1, line, nullptr);
   }
+  // Opens an existing dataset [T, n1, .., nrank] for reading
+  // (ntuple(name, file), ModelG/ntuple.h:108-110); row sizes come from the file.
+  Ntuple(const std::string &name, hid_t file) {
+    H5Api &h = H5Api::get();
+    if (h.Lexists(file, name.c_str(), H5P_DEFAULT) <= 0)
+      throw std::runtime_error("dataset " + name + " does not exist");
+    dset_ = h.Dopen2(file, name.c_str(), H5P_DEFAULT);
+    if (dset_ < 0) throw std::runtime_error("cannot open dataset " + name);
+    space_ = h.Dget_space(dset_);
+    if (h.Sget_simple_extent_ndims(space_) != (int)rank + 1)
+      throw std::runtime_error("dataset " + name + " has rank " +
+                               std::to_string(h.Sget_simple_extent_ndims(space_)) +
+                               ", expected " + std::to_string(rank + 1));
+    hsize_t maxdims[rank + 1];
+    h.Sget_simple_extent_dims(space_, dims_, maxdims);
+    size_t count = 1;
+    for (size_t i = 0; i < rank; i++) {
+      N_[i] = dims_[i + 1];
+      count *= N_[i];
+    }
+    row.assign(count, 0.);
+    hsize_t line[rank + 1];
+    pack(line, 1);
+    mem_ = h.Screate_simple(rank + 1, line, nullptr);
+  }
   ~Ntuple() {
     H5Api &h = H5Api::get();
     h.Sclose(mem_);
@@ -156,6 +185,32 @@ class Ntuple {
     h.Sclose(fs);
   }
   size_t nrows() const { return (size_t)dims_[0]; }
+  std::array<size_t, rank> getN() const {
+    std::array<size_t, rank> n;
+    for (size_t i = 0; i < rank; i++) n[i] = (size_t)N_[i];
+    return n;
+  }
+  // row-major offset of an element of one row (ntuple::at, ntuple.h:98-102)
+  size_t at(const std::array<size_t, rank> &idx) const {
+    size_t k = 0;
+    for (size_t i = 0; i < rank; i++) k = k * N_[i] + idx[i];
+    return k;
+  }
+  double &Row(const std::array<size_t, rank> &idx) { return row[at(idx)]; }
+  // row irow of the file into row (ntuple::readrow, ntuple.h:76-96)
+  void readrow(size_t irow) {
+    H5Api &h = H5Api::get();
+    if (irow >= nrows()) throw std::runtime_error("ntuple::readrow: row out of range");
+    hid_t fs = h.Dget_space(dset_);
+    hsize_t offset[rank + 1] = {};
+    offset[0] = irow;
+    hsize_t count[rank + 1];
+    pack(count, 1);
+    h.Sselect_hyperslab(fs, H5S_SELECT_SET, offset, nullptr, count, nullptr);
+    herr_t e = h.Dread(dset_, h.native_double, mem_, fs, H5P_DEFAULT, row.data());
+    h.Sclose(fs);
+    if (e < 0) throw std::runtime_error("ntuple::readrow: read failed");
+  }
 
  private:
   hsize_t N_[rank];
@@ -184,9 +239,22 @@ class H5File {
     if (id < 0) id = h.Fcreate(name.c_str(), kAccTrunc, H5P_DEFAULT, H5P_DEFAULT);
     if (id < 0) throw std::runtime_error("cannot open " + name);
   }
+  // an existing file, read-only or read-write; throws if it cannot be opened
+  struct Existing {};
+  H5File(const std::string &name, Existing, bool rw) {
+    H5Api &h = H5Api::get();
+    FILE *f = fopen(name.c_str(), "rb");
+    if (f) {
+      fclose(f);
+      id = h.Fopen(name.c_str(), rw ? kAccRdwr : kAccRdonly, H5P_DEFAULT);
+    }
+    if (id < 0) throw std::runtime_error("Unable to open " + name);
+  }
   ~H5File() {
     if (id >= 0) H5Api::get().Fclose(id);
   }
+  H5File(const H5File &) = delete;
+  H5File &operator=(const H5File &) = delete;
 };
 
 // ModelA::write / read: dataset o4fields [N][N][N][10] of the whole field

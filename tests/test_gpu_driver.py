@@ -11,11 +11,13 @@ import subprocess
 import numpy as np
 import pytest
 
+from h5tools import H5DUMP, h5_read
+
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "langevina_amd", "bin", "lgv_superpions")
-H5DUMP = shutil.which("h5dump") or "/opt/conda/bin/h5dump"
+H5COPY = shutil.which("h5copy") or "/opt/conda/bin/h5copy"
 
 PAR = {"NX": 8, "finaltime": 0.4, "initialtime": 0, "deltat": 0.04, "mass0": -4.813,
        "dmassdt": 0.3, "lambda": 1.0, "H": 0.01, "chi": 2.0, "gamma": 1,
@@ -36,17 +38,6 @@ def run_driver(tmp, params, name="in.json"):
     log(r.stdout[-1500:], r.stderr[-1500:])
     assert r.returncode == 0
     return r.stdout
-
-
-def h5_read(path, dset):
-    out = path + "." + dset.strip("/").replace("/", "_") + ".bin"
-    subprocess.run([H5DUMP, "-d", dset, "-b", "LE", "-o", out, path], check=True,
-                   capture_output=True)
-    hdr = subprocess.run([H5DUMP, "-H", "-d", dset, path], check=True, capture_output=True,
-                         text=True).stdout
-    dims = hdr.split("SIMPLE { (")[1].split(")")[0]
-    shape = tuple(int(x) for x in dims.split(","))
-    return np.fromfile(out, dtype="<f8").reshape(shape)
 
 
 def oracle_run(orc, par, nsteps_run, lat=None, thermalize=True):
@@ -106,6 +97,21 @@ def test_driver_layout_and_values(tmp_path, orc):
     np.testing.assert_allclose(fin, lat.u, rtol=1e-10, atol=1e-12)
     s0, s1 = open(os.path.join(tmp, "run_0.rng")).read().split()
     assert int(s0) >= 0 and int(s1) >= 0
+
+    # the offline pipeline of ModelG/x2k.py on this output: x2k of the (empty)
+    # real-space wall, then x2k_rotated of the measured transform
+    from test_x2k import X2K, X2KR, rotated_numpy
+    r = subprocess.run([X2K, h5, "wallz"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out_h5 = os.path.join(tmp, "run_out.h5")
+    assert h5_read(out_h5, "/wallz_k").shape == (0, 11, nk, 2)
+    subprocess.run([H5COPY, "-i", h5, "-o", out_h5, "-s", "/wallz_k", "-d", "/wallz_k_meas"],
+                   check=True, capture_output=True)
+    r = subprocess.run([X2KR, out_h5, "wallz_k_meas"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    rot = h5_read(out_h5, "/wallz_k_meas_rotated")
+    ref = rotated_numpy(wk)
+    np.testing.assert_allclose(rot[..., 0] + 1j * rot[..., 1], ref, rtol=0, atol=1e-13)
 
 
 def test_driver_restart_continues_the_stream(tmp_path, orc):
