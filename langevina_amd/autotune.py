@@ -45,8 +45,11 @@ def autotune_ideal(ctx, dt, steps=2, cache_path=None, verbose=False):
         reps = [r for r, n in enumerate(ctx.N) if n == N]
         if key in cache:
             cap, tz = cache[key][:2]
-            for r in reps:
-                set_ideal_tile(ctx, r, cap, tz)
+            if len(reps) == ctx.nrep:
+                set_ideal_tile(ctx, -1, cap, tz)
+            else:
+                for r in reps:
+                    set_ideal_tile(ctx, r, cap, tz)
             chosen[N] = tuple(cache[key])
             continue
         best = None
@@ -57,8 +60,11 @@ def autotune_ideal(ctx, dt, steps=2, cache_path=None, verbose=False):
                 if tz > N:
                     continue
                 try:
-                    for r in reps:
-                        set_ideal_tile(ctx, r, cap, tz)
+                    if len(reps) == ctx.nrep:   # one plan rebuild for all replicas
+                        set_ideal_tile(ctx, -1, cap, tz)
+                    else:
+                        for r in reps:
+                            set_ideal_tile(ctx, r, cap, tz)
                 except Exception:
                     continue
                 ms = _ideal_ms(ctx, dt, steps)
@@ -66,8 +72,11 @@ def autotune_ideal(ctx, dt, steps=2, cache_path=None, verbose=False):
                     print(f"autotune N={N} cap={cap} tz={tz}: {ms * 1e3:.1f} us", flush=True)
                 if best is None or ms < best[2]:
                     best = (cap, tz, ms)
-        for r in reps:
-            set_ideal_tile(ctx, r, best[0], best[1])
+        if len(reps) == ctx.nrep:
+            set_ideal_tile(ctx, -1, best[0], best[1])
+        else:
+            for r in reps:
+                set_ideal_tile(ctx, r, best[0], best[1])
         chosen[N] = best
         cache[key] = list(best)
     if cache_path:

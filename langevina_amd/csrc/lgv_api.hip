@@ -143,9 +143,6 @@ int build_ideal_plan(lgv_ctx *c) {
     ideal_tile(p.N, p.tile_cap, p.tile_tz, TXH, TY, TZ);
     if ((TY + 2) * (2 * TXH + 2) > kIdealLds)
       return fail(LGV_ERR_ARG, "replica %d: ideal tile %dx%d exceeds the LDS plane", r, TXH, TY);
-    if (2 * TY + 4 * TXH > kBlock)
-      return fail(LGV_ERR_ARG, "replica %d: ideal tile %dx%d has a halo ring of %d sites, "
-                  "more than the %d threads of a block", r, TXH, TY, 2 * TY + 4 * TXH, kBlock);
     work.push_back((int64_t)((p.NH + TXH - 1) / TXH) * ((p.N + TY - 1) / TY) *
                    ((p.N + TZ - 1) / TZ) * kBlock);
   }

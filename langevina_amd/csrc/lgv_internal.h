@@ -9,6 +9,8 @@
 
 namespace lgv {
 
+constexpr int kBlock = 256;  // threads per block of every sweep kernel
+
 // Read-only (for the kernels) description of one replica.  The step
 // coefficients (rdtg, rms, dt_ideal, tau) are computed on the host with the
 // reference's expressions for the dt in use.
@@ -42,7 +44,9 @@ struct ReplicaState {
 
 // Tile of the fused ideal kernel: TXH half-columns x TY rows in x-y,
 // marching over TZ planes in z.  txh_cap (8..64) and tz (4..64) are the
-// tunables (0 = defaults 32 and 16); (TY+2)(2 TXH+2) <= kIdealLds.
+// tunables (0 = defaults 32 and 16); (TY+2)(2 TXH+2) <= kIdealLds.  The
+// one-site halo ring (2 TY + 4 TXH sites) may exceed the block's threads
+// (64 half-columns x 4 rows: 264); its remainder then takes a second pass.
 constexpr int kIdealLds = 800;
 __host__ __device__ inline void ideal_tile(int N, int txh_cap, int tz, int &TXH, int &TY,
                                            int &TZ) {
@@ -50,7 +54,7 @@ __host__ __device__ inline void ideal_tile(int N, int txh_cap, int tz, int &TXH,
   const int cap = txh_cap > 0 ? txh_cap : 32;
   const int zc = tz > 0 ? tz : 16;
   TXH = NH < cap ? NH : cap;
-  TY = 256 / TXH;
+  TY = kBlock / TXH;
   if (TY > N) TY = N;
   TZ = N < zc ? N : zc;
 }
@@ -102,8 +106,6 @@ struct Plan {
 
 // Largest side: N^3 must fit 32-bit site indices.
 constexpr int kMaxN = 1536;
-
-constexpr int kBlock = 256;
 
 // Kernel launchers (lgv_kernels.hip).  fp32 selects the storage type.
 void launch_hb_phi(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,

@@ -590,15 +590,24 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
   const int tx = t % TXH, ty = t / TXH;
   const bool own = ty < th && tx < tw;
   const int xh = xh0 + tx, y = y0 + ty;
-  // halo site of this thread (one-site ring around the tile, no corners)
+  // the one-site ring around the tile (no corners): ring site j = t, t + kBlock, ..
+  // (a second pass only when the ring has more sites than the block threads)
   const int nh = 2 * th + 4 * tw;
-  const bool halo = t < nh;
-  int hlx = 0, hly = 0;
-  if (t < th) { hlx = -1; hly = t; }
-  else if (t < 2 * th) { hlx = 2 * tw; hly = t - th; }
-  else if (t < 2 * th + 2 * tw) { hlx = t - 2 * th; hly = -1; }
-  else { hlx = t - 2 * th - 2 * tw; hly = th; }
-  const int hgx = (2 * xh0 + hlx + N) % N, hgy = (y0 + hly + N) % N;
+  // ring site j: global (x, y) and LDS slot
+  auto ring_site = [&](int j, int &gx, int &gy, int &pos) __attribute__((always_inline)) {
+    int hlx, hly;
+    if (j < th) { hlx = -1; hly = j; }
+    else if (j < 2 * th) { hlx = 2 * tw; hly = j - th; }
+    else if (j < 2 * th + 2 * tw) { hlx = j - 2 * th; hly = -1; }
+    else { hlx = j - 2 * th - 2 * tw; hly = th; }
+    gx = 2 * xh0 + hlx;
+    gx = gx < 0 ? gx + N : (gx >= N ? gx - N : gx);
+    gy = y0 + hly;
+    gy = gy < 0 ? gy + N : (gy >= N ? gy - N : gy);
+    pos = (hly + 1) * LW + (hlx + 1);
+  };
+  int rgx = 0, rgy = 0, rpos = 0;  // this thread's first ring site
+  if (t < nh) ring_site(t, rgx, rgy, rpos);
   const double h = 0.5 * (p.tau_ideal / p.chi);
   const double axx = p.axx, H0 = p.H, dt = p.dt_ideal;
 
@@ -634,12 +643,18 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
 #pragma unroll
         for (int c = 0; c < 4; c++) lds[c][(ty + 1) * LW + (2 * tx + s + 1)] = o[s][c];
     }
-    if (halo) {
+    auto ring_fill = [&](int gx, int gy, int pos) __attribute__((always_inline)) {
       double f[10];
-      load_site<T, W>(p, cur, hgx, hgy, z, f);
+      load_site<T, W>(p, cur, gx, gy, z, f);
       rotate_site_fast(f, f + 4, f + 7, h);
 #pragma unroll
-      for (int c = 0; c < 4; c++) lds[c][(hly + 1) * LW + (hlx + 1)] = f[c];
+      for (int c = 0; c < 4; c++) lds[c][pos] = f[c];
+    };
+    if (t < nh) ring_fill(rgx, rgy, rpos);
+    for (int j = t + kBlock; j < nh; j += kBlock) {
+      int gx, gy, pos;
+      ring_site(j, gx, gy, pos);
+      ring_fill(gx, gy, pos);
     }
   };
 

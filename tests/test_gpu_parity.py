@@ -227,17 +227,27 @@ def test_ideal_tile_does_not_change_results(L, orc, N):
             np.testing.assert_array_equal(got, ref)
 
 
-def test_ideal_tile_rejects_a_ring_larger_than_a_block(L):
-    """At N >= 128 a 64-half-column tile is 4 rows high: its halo ring has
-    2*4 + 4*64 = 264 sites, more than the 256 threads that compute it."""
-    log("tile ring check")
+def test_ideal_tile_two_pass_ring(L):
+    """At N = 128 a 64-half-column tile is 4 rows high: its halo ring of
+    2*4 + 4*64 = 264 sites is more than the 256 threads of a block, and the
+    remainder takes a second pass.  It gives the bits of the default tile,
+    also with a chunk length that leaves a short last chunk."""
+    log("tile two-pass ring")
     from langevina_amd.autotune import set_ideal_tile
-    ctx = L.Context(L.model_params(128, seed=3, **PAR))
-    with pytest.raises(L.LgvError, match="halo ring"):
-        set_ideal_tile(ctx, 0, 64, 16)
-    set_ideal_tile(ctx, 0, 32, 16)  # the context stays usable
-    ctx.step(0.04)
-    ctx.close()
+    N = 128
+    u = np.random.default_rng(4).normal(size=(N, N, N, 10)) * 0.5
+    ref = None
+    for cap, tz in ((32, 16), (64, 16), (64, 5)):
+        ctx = L.Context(L.model_params(N, seed=3, **PAR))
+        set_ideal_tile(ctx, 0, cap, tz)
+        ctx.set_field(u)
+        ctx.step(0.04)
+        got = ctx.get_field()
+        ctx.close()
+        if ref is None:
+            ref = got
+        else:
+            np.testing.assert_array_equal(got, ref, err_msg=f"cap {cap} tz {tz}")
 
 
 @pytest.mark.parametrize("variant", ["nodiffuse", "onlydiffuse"])
