@@ -225,12 +225,19 @@ struct Timed {
       }
     }
     hipEvent_t e0 = c->ev_pool[c->ev_used++];
+    if (hipEventRecord(e0, c->stream) != hipSuccess) {  // timing is best effort
+      c->timing = false;
+      return;
+    }
     e1 = c->ev_pool[c->ev_used++];
-    hipEventRecord(e0, c->stream);
     c->ev_pairs.push_back({cls, {e0, e1}});
   }
   ~Timed() {
-    if (c->timing && e1) hipEventRecord(e1, c->stream);
+    if (!e1) return;
+    if (hipEventRecord(e1, c->stream) != hipSuccess) {
+      c->timing = false;
+      c->ev_pairs.pop_back();  // the pair this object pushed: never resolved
+    }
   }
 };
 
@@ -438,30 +445,31 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
 
 int lgv_destroy(lgv_ctx *c) {
   if (!c) return LGV_OK;
-  hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
-  if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
-  for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
-  for (void *f : c->fields) hipFree(f);
-  hipFree(c->dP);
-  hipFree(c->dS);
+  // releases everything even after a failure and reports the first one
+  hipError_t first = hipSuccess;
+  auto note = [&](hipError_t e) {
+    if (first == hipSuccess && e != hipSuccess) first = e;
+  };
+  note(hipSetDevice(c->device));
+  if (c->stream) note(hipStreamSynchronize(c->stream));
+  if (c->graph_exec) note(hipGraphExecDestroy(c->graph_exec));
+  for (hipEvent_t e : c->ev_pool) note(hipEventDestroy(e));
+  for (void *f : c->fields) note(hipFree(f));
+  note(hipFree(c->dP));
+  note(hipFree(c->dS));
   for (Plan *pl : {&c->plan_half, &c->plan_full, &c->plan_ideal, &c->plan_charge, &c->plan_hb}) {
-    hipFree(pl->d_blk_rep);
-    hipFree(pl->d_blk_first);
+    note(hipFree(pl->d_blk_rep));
+    note(hipFree(pl->d_blk_first));
   }
-  hipFree(c->mon_phi);
-  hipFree(c->mon_charge);
-  hipFree(c->mon_ideal);
-  hipFree(c->eslots);
-  hipFree(c->d_aos);
-  hipFree(c->d_partXZ);
-  hipFree(c->d_partXY);
-  hipFree(c->d_wall);
-  hipFree(c->d_wallphase);
-  hipFree(c->d_wallk);
-  hipFree(c->d_wallphasek);
-  if (c->stream) hipStreamDestroy(c->stream);
+  for (void *q : {(void *)c->mon_phi, (void *)c->mon_charge, (void *)c->mon_ideal,
+                  (void *)c->eslots, (void *)c->d_aos, (void *)c->d_partXZ, (void *)c->d_partXY,
+                  (void *)c->d_wall, (void *)c->d_wallphase, (void *)c->d_wallk,
+                  (void *)c->d_wallphasek})
+    note(hipFree(q));
+  if (c->stream) note(hipStreamDestroy(c->stream));
   delete c;
+  if (first != hipSuccess)
+    return fail(LGV_ERR_HIP, "lgv_destroy: %s", hipGetErrorString(first));
   return LGV_OK;
 }
 

@@ -308,7 +308,11 @@ int lgv_em_create(const lgv_em_params *p, int nsteps, int ckpt_every, int device
 
 int lgv_em_set_stream(lgv_em *h, void *stream) {
   if (!h) return em_fail(LGV_ERR_ARG, "null handle");
-  if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
+  if (h->own_stream && h->stream) {
+    EMCHECK(hipStreamSynchronize(h->stream));
+    EMCHECK(hipStreamDestroy(h->stream));
+    h->stream = nullptr;
+  }
   h->own_stream = stream == nullptr;
   if (stream) {
     h->stream = (hipStream_t)stream;
@@ -406,17 +410,20 @@ int lgv_em_backward(lgv_em *h, double *grad_phi0, double grad_params[3]) {
 
 int lgv_em_destroy(lgv_em *h) {
   if (!h) return LGV_OK;
-  hipSetDevice(h->device);
-  if (h->stream) hipStreamSynchronize(h->stream);
-  hipFree(h->ckpt);
-  hipFree(h->seg);
-  hipFree(h->work[0]);
-  hipFree(h->work[1]);
-  hipFree(h->part);
-  hipFree(h->phibar);
-  hipFree(h->gslots);
-  if (h->own_stream && h->stream) hipStreamDestroy(h->stream);
+  // releases everything even after a failure and reports the first one
+  hipError_t first = hipSuccess;
+  auto note = [&](hipError_t e) {
+    if (first == hipSuccess && e != hipSuccess) first = e;
+  };
+  note(hipSetDevice(h->device));
+  if (h->stream) note(hipStreamSynchronize(h->stream));
+  for (void *q : {(void *)h->ckpt, (void *)h->seg, (void *)h->work[0], (void *)h->work[1],
+                  (void *)h->part, (void *)h->phibar, (void *)h->gslots})
+    note(hipFree(q));
+  if (h->own_stream && h->stream) note(hipStreamDestroy(h->stream));
   delete h;
+  if (first != hipSuccess)
+    return em_fail(LGV_ERR_HIP, "lgv_em_destroy: %s", hipGetErrorString(first));
   return LGV_OK;
 }
 
