@@ -81,6 +81,14 @@ __device__ void block_ksum_store(KSum k, double *out2) {
   }
 }
 
+// Zeroes n doubles.  A kernel rather than hipMemsetAsync, so that a caller
+// capturing the backward pass into a graph replays a plain kernel node.
+__global__ void k_em_zero(double *__restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.;
+}
+
 // One forward step in -> out; per-block compensated partial of sum out[0].
 __global__ __launch_bounds__(kEmBlock) void k_em_step(const double *__restrict__ in,
                                                       double *__restrict__ out, int N,
@@ -366,8 +374,11 @@ int lgv_em_backward(lgv_em *h, double *grad_phi0, double grad_params[3]) {
   mean /= W;
   const double gdt = h->p.gamma * h->p.dt;
   double *lam = h->work[0], *nxt = h->work[1];
-  EMCHECK(hipMemsetAsync(lam, 0, st, h->stream));
-  EMCHECK(hipMemsetAsync(h->gslots, 0, h->nblocks * sizeof(GradSlot), h->stream));
+  static_assert(sizeof(GradSlot) == 6 * sizeof(double), "GradSlot is six doubles");
+  k_em_zero<<<1024, 256, 0, h->stream>>>(lam, 4 * h->V);
+  k_em_zero<<<64, 256, 0, h->stream>>>(reinterpret_cast<double *>(h->gslots),
+                                       (int64_t)6 * h->nblocks);
+  EMCHECK(hipGetLastError());
   int rc;
   for (int j = h->nseg - 1; j >= 0; j--) {
     const int n0 = j * h->K, n1 = std::min(h->nsteps, n0 + h->K);
