@@ -412,13 +412,22 @@ __device__ __forceinline__ void charge_line(const ReplicaParams &p, int cur, int
   Xoro gA = xoro_seeded(site_key(k2, lin[1]));
   uint64_t c0 = 0;
   double sd0 = 0., su0 = 0.;
+  // every load of the line before the first store (see charge_column)
+  double w[6][4];
 #pragma unroll
   for (int L = 4; L < 10; L++) {
     const GPlane<T, W> f1 = gplane<T, W>(p, cur, L, e1), f2 = gplane<T, W>(p, cur, L, e2);
-    const double vL = f1.ld(ii[0]);
-    const double vA = f2.ld(ii[1]);
-    const double vB = f1.ld(ii[2]);
-    const double vR = f2.ld(ii[3]);
+    w[L - 4][0] = f1.ld(ii[0]);
+    w[L - 4][1] = f2.ld(ii[1]);
+    w[L - 4][2] = f1.ld(ii[2]);
+    w[L - 4][3] = f2.ld(ii[3]);
+  }
+#pragma unroll
+  for (int L = 4; L < 10; L++) {
+    const double vL = w[L - 4][0];
+    const double vA = w[L - 4][1];
+    const double vB = w[L - 4][2];
+    const double vR = w[L - 4][3];
     double tL, a1, b1, tR;
     charge_pair(vL, vA, ck, gL, tL, a1, cnt, dSd, dSu, true);
     charge_pair(vB, vR, ck, gB, b1, tR, c0, sd0, su0, false);
@@ -477,10 +486,19 @@ __device__ __forceinline__ void charge_column(const ReplicaParams &p, int cur, i
       at(m0, i0, l0);
       g2 = xoro_seeded(site_key(k2, l0));
     }
+    // every load of the pair is issued before the first store: the compiler
+    // cannot prove that the stores (other buffer) miss the loaded planes and
+    // would otherwise wait for each component's load after the previous store
+    double w1[6], w2[6];
 #pragma unroll
     for (int L = 4; L < 10; L++) {
-      const double v1 = gplane<T, W>(p, cur, L, e1).ld(i1);
-      const double v2 = gplane<T, W>(p, cur, L, e2).ld(i2);
+      w1[L - 4] = gplane<T, W>(p, cur, L, e1).ld(i1);
+      w2[L - 4] = gplane<T, W>(p, cur, L, e2).ld(i2);
+    }
+#pragma unroll
+    for (int L = 4; L < 10; L++) {
+      const double v1 = w1[L - 4];
+      const double v2 = w2[L - 4];
       double a, b;
       charge_pair(v1, v2, ck, g1, a, b, cnt, dSd, dSu, own1);
       a = (double)(T)a;
