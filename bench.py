@@ -2,8 +2,8 @@
 """Throughput of the ModelG time step (PV2HBSplit23) on one MI355X.
 
     python bench.py [--steps K] [--warmup W] [--size 256] [--precision fp64|fp32]
-                    [--graph] [--batched] [--drift] [--no-cpu-baseline] [--calibrate-copy]
-                    [--no-autotune | --ideal-tile CAP TZ]
+                    [--graph] [--batched] [--drift] [--em] [--no-cpu-baseline]
+                    [--calibrate-copy] [--no-autotune | --ideal-tile CAP TZ]
 
 One "step" is PV2HBSplit::step(dt) (3 ideal substeps, 12 phi Metropolis
 half-sweeps, 6 charge sub-sweeps; here 3 + 12 + 3 kernel launches) applied to
@@ -146,6 +146,60 @@ def batched_replicas(L):
     return reps
 
 
+# Model A Euler-Maruyama (lgv_em_*): algorithmic bytes per site-step (FP64,
+# 4 components): forward reads phi and writes phi (64); the reverse pass
+# recomputes each checkpoint segment (64) and runs the adjoint stencil, which
+# reads phi_n and lambda and writes lambda (96).
+EM_FWD_BYTES, EM_BWD_BYTES = 64, 64 + 96
+
+
+def bench_em(args, N=128, nsteps=500, ckpt=50, window=250, reps=3):
+    """Forward + reverse of the Euler-Maruyama loss at 128^3 x 500 steps
+    (checkpoints every 50): wall time per call with the inputs on the device,
+    best of reps after one untimed call."""
+    import torch
+    from langevina_amd.em import EulerMaruyama
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device="cpu").manual_seed(42)
+    phi0 = (0.1 * torch.randn(4, N, N, N, generator=g, dtype=torch.float64)).to(dev)
+    grad = torch.empty_like(phi0)
+    em = EulerMaruyama(N, m2=-4.813, lam=1.0, H=0.01, gamma=1.0, dt=0.02, seed=42,
+                       nsteps=nsteps, ckpt_every=ckpt)
+    em.set_stream(torch.cuda.current_stream().cuda_stream)
+    fwd, bwd = [], []
+    for i in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loss, _ = em.forward(phi0.data_ptr(), window)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        gp = em.backward(grad.data_ptr())
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if i:
+            fwd.append(t1 - t0)
+            bwd.append(t2 - t1)
+    em.close()
+    sites = N ** 3
+    f, b = min(fwd), min(bwd)
+    out = {
+        "metric": "Model A Euler-Maruyama site-steps/s (forward; reverse with checkpoints)",
+        "value": sites * nsteps / f, "unit": "site-steps/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "f64",
+        "data": "synthetic (phi0 ~ N(0, 0.1^2), seed 42)",
+        "config": {"workload": f"Model A EM {N}^3 x {nsteps} steps, checkpoints every {ckpt}, "
+                               f"loss window {window}", "m2": -4.813, "lambda": 1.0, "H": 0.01,
+                   "dt": 0.02},
+        "forward_ms": f * 1e3, "backward_ms": b * 1e3,
+        "forward_alg_GBps": sites * nsteps * EM_FWD_BYTES / f / 1e9,
+        "backward_alg_GBps": sites * nsteps * EM_BWD_BYTES / b / 1e9,
+        "forward_roofline_frac": sites * nsteps * EM_FWD_BYTES / f / HBM_PEAK,
+        "backward_roofline_frac": sites * nsteps * EM_BWD_BYTES / b / HBM_PEAK,
+        "loss": loss, "grad_params": [float(x) for x in gp],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_batched(args):
     import langevina_amd as L
     from langevina_amd.autotune import autotune_ideal
@@ -277,6 +331,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the copy kernel used to scale HBM counters")
+    ap.add_argument("--em", action="store_true",
+                    help="Model A Euler-Maruyama forward/reverse at 128^3 x 500 steps")
     ap.add_argument("--drift", action="store_true",
                     help="report FP32-field observables against FP64 (500 steps)")
     args = ap.parse_args()
@@ -289,6 +345,9 @@ def main():
         return
     if args.drift:
         precision_drift(args)
+        return
+    if args.em:
+        bench_em(args)
         return
     if args.batched:
         bench_batched(args)

@@ -634,6 +634,10 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
   // three waves per SIMD (measured faster for FP32, slower for FP64, whose
   // re-reads miss L2)
   constexpr bool kCarry = sizeof(T) == 8;
+  // FP64 fields also load the raw fields of the thread's first ring site of
+  // plane z+1 before the kick of plane z, so that their latency overlaps the
+  // kick (free at FP64's 2 waves per SIMD; FP32 would drop from 3 waves to 2)
+  constexpr bool kEarlyRing = sizeof(T) == 8;
   double cen[2][4], nex[2][4];  // rotated phi at the own sites of planes z, z+1
   double chg[2][6], chn[2][6];  // raw charges of planes z, z+1 (kCarry only)
   auto rotate_own = [&](int z, double out[2][4], double ch[2][6]) __attribute__((always_inline)) {
@@ -668,7 +672,7 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
 #pragma unroll
       for (int c = 0; c < 4; c++) lds[c][pos] = f[c];
     };
-    if (t < nh) ring_fill(rgx, rgy, rpos);
+    if (!kEarlyRing && t < nh) ring_fill(rgx, rgy, rpos);
     for (int j = t + kBlock; j < nh; j += kBlock) {
       int gx, gy, pos;
       ring_site(j, gx, gy, pos);
@@ -684,11 +688,21 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
       for (int c = 0; c < 4; c++) lprv[s][c][t] = nex[s][c];
   }
   rotate_own(z0, cen, chg);
+  if (kEarlyRing && t < nh) {
+    double f[10];
+    load_site<T, W>(p, cur, rgx, rgy, z0, f);
+    rotate_site_fast(f, f + 4, f + 7, h);
+#pragma unroll
+    for (int c = 0; c < 4; c++) lds[c][rpos] = f[c];
+  }
   fill_lds(z0, cen);
   for (int k = 0; k < tz; k++) {
     const int z = z0 + k;
     const int zn = (z + 1) % N;
     rotate_own(zn, nex, chn);
+    double rf[10];
+    const bool ring1 = kEarlyRing && t < nh && k + 1 < tz;
+    if (ring1) load_site<T, W>(p, cur, rgx, rgy, zn, rf);
     __syncthreads();
     if (own) {
       const uint32_t i = ((uint32_t)z * N + y) * NH + xh;
@@ -753,6 +767,11 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
       }
     }
     __syncthreads();
+    if (ring1) {
+      rotate_site_fast(rf, rf + 4, rf + 7, h);
+#pragma unroll
+      for (int c = 0; c < 4; c++) lds[c][rpos] = rf[c];
+    }
     if (k + 1 < tz) fill_lds(zn, nex);
 #pragma unroll
     for (int s = 0; s < 2; s++) {
