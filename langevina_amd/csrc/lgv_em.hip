@@ -39,11 +39,14 @@ struct GradSlot {
 __device__ __forceinline__ int wp(int a, int N) { return a == N ? 0 : a; }
 __device__ __forceinline__ int wm(int a, int N) { return a < 0 ? N - 1 : a; }
 
+// (x, y, z) of site s = (z N + y) N + x; 32-bit division (s < N^3 <= 2^30),
+// several times cheaper than the 64-bit one
 __device__ __forceinline__ void site_xyz(int64_t s, int N, int &x, int &y, int &z) {
-  x = (int)(s % N);
-  const int64_t yz = s / N;
-  y = (int)(yz % N);
-  z = (int)(yz / N);
+  const uint32_t s32 = (uint32_t)s, n = (uint32_t)N;
+  const uint32_t yz = s32 / n;
+  x = (int)(s32 - yz * n);
+  z = (int)(yz / n);
+  y = (int)(yz - (uint32_t)z * n);
 }
 
 __device__ __forceinline__ void em_noise(uint64_t skey, int64_t s, int kind, double xi[4]) {
@@ -125,16 +128,27 @@ __global__ __launch_bounds__(kEmBlock) void k_em_step(const double *__restrict__
 }
 
 // phibar[n] = (sum of the block partials) / V
-__global__ void k_em_phibar(const double *part, int nblocks, int64_t V, double *phibar, int n) {
+// phibar[n] = (compensated sum of the nblocks partials) / V, one block.
+constexpr int kPhibarThreads = 1024;
+__global__ __launch_bounds__(kPhibarThreads) void k_em_phibar(const double *part, int nblocks,
+                                                              int64_t V, double *phibar, int n) {
   KSum acc{0., 0.};
-  for (int b = threadIdx.x; b < nblocks; b += 64) ksum_merge(acc, KSum{part[2 * b], part[2 * b + 1]});
+  for (int b = threadIdx.x; b < nblocks; b += kPhibarThreads)
+    ksum_merge(acc, KSum{part[2 * b], part[2 * b + 1]});
   for (int o = 32; o > 0; o >>= 1) {
     KSum t;
     t.s = __shfl_xor(acc.s, o, 64);
     t.c = __shfl_xor(acc.c, o, 64);
     ksum_merge(acc, t);
   }
-  if (threadIdx.x == 0) phibar[n] = ksum_value(acc) / (double)V;
+  __shared__ KSum sh[kPhibarThreads / 64];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    KSum a = sh[0];
+    for (int i = 1; i < kPhibarThreads / 64; i++) ksum_merge(a, sh[i]);
+    phibar[n] = ksum_value(a) / (double)V;
+  }
 }
 
 // One adjoint step: lam_out = J(phi)^T (lam_in + dl e0); accumulates the
@@ -238,8 +252,8 @@ struct lgv_em {
 
 namespace {
 
-// Steps n0..n1-1 starting from `start`, recording phibar of every step.
-// Forward mode (states == nullptr): ping-pongs through the work buffers and
+// Steps n0..n1-1 starting from `start`.  Forward mode (states == nullptr):
+// records phibar of every step, ping-pongs through the work buffers and
 // leaves phi_{n1} in `last`.  Recompute mode: states[i] = phi_{n0+i} for
 // i = 0..n1-n0 (start is copied to states[0]); the work buffers, which hold
 // the adjoint state during the backward pass, are not touched.
@@ -260,7 +274,8 @@ int em_run_segment(lgv_em *h, const double *start, int n0, int n1, double *state
     k_em_step<<<h->nblocks, kEmBlock, 0, h->stream>>>(in, out, h->p.N, h->p.m2, h->p.lambda,
                                                       h->p.H, gdt, amp, skey, h->p.noise_kind,
                                                       h->part);
-    k_em_phibar<<<1, 64, 0, h->stream>>>(h->part, h->nblocks, h->V, h->phibar, n);
+    if (!states)  // the reverse pass uses the forward's phibar
+      k_em_phibar<<<1, kPhibarThreads, 0, h->stream>>>(h->part, h->nblocks, h->V, h->phibar, n);
     in = out;
   }
   if (!states && n1 == n0 && last != start)
