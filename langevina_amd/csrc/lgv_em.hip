@@ -32,22 +32,26 @@ namespace {
 constexpr int kEmBlock = 256;
 constexpr uint64_t kEmTag = 0x4000000000000000ULL;
 
+// Tile of the two stencil marches: TX sites in x by TY rows in y, chunks of
+// TZ planes in z.  A block's plane (tile plus a one-site ring, no corners)
+// sits in LDS; each thread keeps its own column's planes z-1, z, z+1 in
+// registers.
+constexpr int kEmGrid = 400;  // (TY+2)(TX+2) <= 6 * 66
+__host__ __device__ inline void em_tile(int N, int &TX, int &TY, int &TZ) {
+  TX = N < 64 ? N : 64;
+  TY = kEmBlock / TX;
+  if (TY > N) TY = N;
+  TZ = N < 8 ? N : 8;
+}
+__host__ __device__ inline int em_blocks(int N) {
+  int TX, TY, TZ;
+  em_tile(N, TX, TY, TZ);
+  return ((N + TX - 1) / TX) * ((N + TY - 1) / TY) * ((N + TZ - 1) / TZ);
+}
+
 struct GradSlot {
   double m2_s, m2_c, h_s, h_c, l_s, l_c;
 };
-
-__device__ __forceinline__ int wp(int a, int N) { return a == N ? 0 : a; }
-__device__ __forceinline__ int wm(int a, int N) { return a < 0 ? N - 1 : a; }
-
-// (x, y, z) of site s = (z N + y) N + x; 32-bit division (s < N^3 <= 2^30),
-// several times cheaper than the 64-bit one
-__device__ __forceinline__ void site_xyz(int64_t s, int N, int &x, int &y, int &z) {
-  const uint32_t s32 = (uint32_t)s, n = (uint32_t)N;
-  const uint32_t yz = s32 / n;
-  x = (int)(s32 - yz * n);
-  z = (int)(yz / n);
-  y = (int)(yz - (uint32_t)z * n);
-}
 
 __device__ __forceinline__ void em_noise(uint64_t skey, int64_t s, int kind, double xi[4]) {
   Xoro g = xoro_seeded(site_key(skey, (uint64_t)s));
@@ -92,42 +96,114 @@ __global__ void k_em_zero(double *__restrict__ p, int64_t n) {
     p[i] = 0.;
 }
 
-// One forward step in -> out; per-block compensated partial of sum out[0].
+// Geometry of one march block: tile origin and extent, this thread's column
+// and its ring site.
+struct EmMarch {
+  int N, x0, y0, z0, tw, th, tz, GW;
+  int tx, ty, x, y;
+  bool own, halo;
+  int hx, hy, hpos, pos;
+  __device__ explicit EmMarch(int n) : N(n) {
+    int TX, TY, TZ;
+    em_tile(N, TX, TY, TZ);
+    const int ntx = (N + TX - 1) / TX, nty = (N + TY - 1) / TY;
+    int b = blockIdx.x;
+    const int bx = b % ntx;
+    b /= ntx;
+    const int by = b % nty, bz = b / nty;
+    x0 = bx * TX;
+    y0 = by * TY;
+    z0 = bz * TZ;
+    tw = min(TX, N - x0);
+    th = min(TY, N - y0);
+    tz = min(TZ, N - z0);
+    GW = tw + 2;
+    const int t = threadIdx.x;
+    tx = t % TX;
+    ty = t / TX;
+    own = t < TX * TY && tx < tw && ty < th;
+    x = x0 + tx;
+    y = y0 + ty;
+    pos = (ty + 1) * GW + (tx + 1);
+    halo = t < 2 * th + 2 * tw;
+    int hlx, hly;
+    if (t < th) { hlx = -1; hly = t; }
+    else if (t < 2 * th) { hlx = tw; hly = t - th; }
+    else if (t < 2 * th + tw) { hlx = t - 2 * th; hly = -1; }
+    else { hlx = t - 2 * th - tw; hly = th; }
+    hx = x0 + hlx;
+    hx = hx < 0 ? hx + N : (hx >= N ? hx - N : hx);
+    hy = y0 + hly;
+    hy = hy < 0 ? hy + N : (hy >= N ? hy - N : hy);
+    hpos = (hly + 1) * GW + (hlx + 1);
+  }
+  __device__ uint32_t site(int xx, int yy, int z) const {
+    return ((uint32_t)z * N + yy) * N + xx;
+  }
+};
+
+// One forward step in -> out, marching z with the plane in LDS; per-block
+// compensated partial of sum out[0].
 __global__ __launch_bounds__(kEmBlock) void k_em_step(const double *__restrict__ in,
                                                       double *__restrict__ out, int N,
                                                       double m2, double lam, double H,
                                                       double gdt, double amp, uint64_t skey,
                                                       int kind, double *__restrict__ part) {
+  __shared__ double Q[2][4][kEmGrid];
   const int64_t V = (int64_t)N * N * N;
-  const int64_t s = (int64_t)blockIdx.x * kEmBlock + threadIdx.x;
+  const EmMarch m(N);
   KSum acc{0., 0.};
-  if (s < V) {
-    int x, y, z;
-    site_xyz(s, N, x, y, z);
-    const int64_t ixp = s - x + wp(x + 1, N), ixm = s - x + wm(x - 1, N);
-    const int64_t iyp = s + (int64_t)(wp(y + 1, N) - y) * N, iym = s + (int64_t)(wm(y - 1, N) - y) * N;
-    const int64_t izp = s + (int64_t)(wp(z + 1, N) - z) * N * N;
-    const int64_t izm = s + (int64_t)(wm(z - 1, N) - z) * N * N;
-    double c[4];
-    for (int a = 0; a < 4; a++) c[a] = in[a * V + s];
-    const double s2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2] + c[3] * c[3];
-    double xi[4];
-    em_noise(skey, s, kind, xi);
+  double cm[4], cc[4], cp[4];  // own column at z-1, z, z+1
+  if (m.own) {
+    const int zm = m.z0 == 0 ? N - 1 : m.z0 - 1;
+#pragma unroll
     for (int a = 0; a < 4; a++) {
-      const double *f = in + a * V;
-      const double nb = (f[ixp] + f[ixm]) + (f[iyp] + f[iym]) + (f[izp] + f[izm]);
-      const double lap = nb - 6.0 * c[a];
-      double drift = (-lap + m2 * c[a]) + (lam * s2) * c[a];
-      if (a == 0) drift = drift - H;
-      const double o = (c[a] - gdt * drift) + amp * xi[a];
-      out[a * V + s] = o;
-      if (a == 0) acc.s = o;
+      cm[a] = in[a * V + m.site(m.x, m.y, zm)];
+      cc[a] = in[a * V + m.site(m.x, m.y, m.z0)];
+    }
+  }
+  for (int j = 0; j < m.tz; j++) {
+    const int z = m.z0 + j;
+    const int zp = z + 1 == N ? 0 : z + 1;
+    double(*g)[kEmGrid] = Q[j & 1];
+    if (m.own) {
+#pragma unroll
+      for (int a = 0; a < 4; a++) {
+        cp[a] = in[a * V + m.site(m.x, m.y, zp)];
+        g[a][m.pos] = cc[a];
+      }
+    }
+    if (m.halo) {
+#pragma unroll
+      for (int a = 0; a < 4; a++) g[a][m.hpos] = in[a * V + m.site(m.hx, m.hy, z)];
+    }
+    __syncthreads();
+    if (m.own) {
+      const uint32_t s = m.site(m.x, m.y, z);
+      const double s2 = cc[0] * cc[0] + cc[1] * cc[1] + cc[2] * cc[2] + cc[3] * cc[3];
+      double xi[4];
+      em_noise(skey, s, kind, xi);
+#pragma unroll
+      for (int a = 0; a < 4; a++) {
+        const double nb = (g[a][m.pos + 1] + g[a][m.pos - 1]) +
+                          (g[a][m.pos + m.GW] + g[a][m.pos - m.GW]) + (cp[a] + cm[a]);
+        const double lap = nb - 6.0 * cc[a];
+        double drift = (-lap + m2 * cc[a]) + (lam * s2) * cc[a];
+        if (a == 0) drift = drift - H;
+        const double o = (cc[a] - gdt * drift) + amp * xi[a];
+        out[a * V + s] = o;
+        if (a == 0) ksum_add(acc, o);
+      }
+#pragma unroll
+      for (int a = 0; a < 4; a++) {
+        cm[a] = cc[a];
+        cc[a] = cp[a];
+      }
     }
   }
   block_ksum_store(acc, part + 2 * blockIdx.x);
 }
 
-// phibar[n] = (sum of the block partials) / V
 // phibar[n] = (compensated sum of the nblocks partials) / V, one block.
 constexpr int kPhibarThreads = 1024;
 __global__ __launch_bounds__(kPhibarThreads) void k_em_phibar(const double *part, int nblocks,
@@ -151,64 +227,93 @@ __global__ __launch_bounds__(kPhibarThreads) void k_em_phibar(const double *part
   }
 }
 
-// One adjoint step: lam_out = J(phi)^T (lam_in + dl e0); accumulates the
-// parameter gradients into the block's slot.
+// One adjoint step: lam_out = J(phi)^T (lam_in + dl e0), marching z with
+// lam_in's plane in LDS; accumulates the parameter gradients into the
+// block's slot.
 __global__ __launch_bounds__(kEmBlock) void k_em_adjoint(const double *__restrict__ phi,
                                                          const double *__restrict__ lin,
                                                          double *__restrict__ lout, int N,
                                                          double m2, double lam, double gdt,
                                                          double dl,
                                                          GradSlot *__restrict__ slots) {
+  __shared__ double Q[2][4][kEmGrid];
   const int64_t V = (int64_t)N * N * N;
-  const int64_t s = (int64_t)blockIdx.x * kEmBlock + threadIdx.x;
-  double gm2 = 0., gl = 0., gh = 0.;
-  if (s < V) {
-    int x, y, z;
-    site_xyz(s, N, x, y, z);
-    const int64_t ixp = s - x + wp(x + 1, N), ixm = s - x + wm(x - 1, N);
-    const int64_t iyp = s + (int64_t)(wp(y + 1, N) - y) * N, iym = s + (int64_t)(wm(y - 1, N) - y) * N;
-    const int64_t izp = s + (int64_t)(wp(z + 1, N) - z) * N * N;
-    const int64_t izm = s + (int64_t)(wm(z - 1, N) - z) * N * N;
-    double c[4], l[4];
+  const EmMarch m(N);
+  KSum ga{0., 0.}, gb{0., 0.}, gc{0., 0.};  // d/d m2, d/d H, d/d lambda
+  double lm[4], lc[4], lp[4];  // lam_in of the own column at z-1, z, z+1
+  if (m.own) {
+    const int zm = m.z0 == 0 ? N - 1 : m.z0 - 1;
+#pragma unroll
     for (int a = 0; a < 4; a++) {
-      c[a] = phi[a * V + s];
-      l[a] = lin[a * V + s] + (a == 0 ? dl : 0.0);
+      lm[a] = lin[a * V + m.site(m.x, m.y, zm)];
+      lc[a] = lin[a * V + m.site(m.x, m.y, m.z0)];
     }
-    const double s2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2] + c[3] * c[3];
-    const double pl = c[0] * l[0] + c[1] * l[1] + c[2] * l[2] + c[3] * l[3];
-    for (int a = 0; a < 4; a++) {
-      const double *f = lin + a * V;
-      const double e = a == 0 ? dl : 0.0;
-      const double nb = ((f[ixp] + e) + (f[ixm] + e)) + ((f[iyp] + e) + (f[iym] + e)) +
-                        ((f[izp] + e) + (f[izm] + e));
-      const double lapl = nb - 6.0 * l[a];
-      const double jt = (-lapl + m2 * l[a]) + lam * (s2 * l[a] + 2.0 * c[a] * pl);
-      lout[a * V + s] = l[a] - gdt * jt;
-    }
-    gm2 = -gdt * pl;
-    gl = -gdt * s2 * pl;
-    gh = gdt * l[0];
   }
-  KSum a{gm2, 0.}, b{gh, 0.}, cc{gl, 0.};
+  for (int j = 0; j < m.tz; j++) {
+    const int z = m.z0 + j;
+    const int zp = z + 1 == N ? 0 : z + 1;
+    double(*g)[kEmGrid] = Q[j & 1];
+    double c[4];
+    if (m.own) {
+      const uint32_t s = m.site(m.x, m.y, z);
+#pragma unroll
+      for (int a = 0; a < 4; a++) {
+        lp[a] = lin[a * V + m.site(m.x, m.y, zp)];
+        c[a] = phi[a * V + s];
+        g[a][m.pos] = lc[a];
+      }
+    }
+    if (m.halo) {
+#pragma unroll
+      for (int a = 0; a < 4; a++) g[a][m.hpos] = lin[a * V + m.site(m.hx, m.hy, z)];
+    }
+    __syncthreads();
+    if (m.own) {
+      const uint32_t s = m.site(m.x, m.y, z);
+      double l[4];
+#pragma unroll
+      for (int a = 0; a < 4; a++) l[a] = lc[a] + (a == 0 ? dl : 0.0);
+      const double s2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2] + c[3] * c[3];
+      const double pl = c[0] * l[0] + c[1] * l[1] + c[2] * l[2] + c[3] * l[3];
+#pragma unroll
+      for (int a = 0; a < 4; a++) {
+        const double e = a == 0 ? dl : 0.0;
+        const double nb = ((g[a][m.pos + 1] + e) + (g[a][m.pos - 1] + e)) +
+                          ((g[a][m.pos + m.GW] + e) + (g[a][m.pos - m.GW] + e)) +
+                          ((lp[a] + e) + (lm[a] + e));
+        const double lapl = nb - 6.0 * l[a];
+        const double jt = (-lapl + m2 * l[a]) + lam * (s2 * l[a] + 2.0 * c[a] * pl);
+        lout[a * V + s] = l[a] - gdt * jt;
+      }
+      ksum_add(ga, -gdt * pl);
+      ksum_add(gb, gdt * l[0]);
+      ksum_add(gc, -gdt * s2 * pl);
+#pragma unroll
+      for (int a = 0; a < 4; a++) {
+        lm[a] = lc[a];
+        lc[a] = lp[a];
+      }
+    }
+  }
   for (int o = 32; o > 0; o >>= 1) {
     KSum t;
-    t.s = __shfl_xor(a.s, o, 64); t.c = __shfl_xor(a.c, o, 64); ksum_merge(a, t);
-    t.s = __shfl_xor(b.s, o, 64); t.c = __shfl_xor(b.c, o, 64); ksum_merge(b, t);
-    t.s = __shfl_xor(cc.s, o, 64); t.c = __shfl_xor(cc.c, o, 64); ksum_merge(cc, t);
+    t.s = __shfl_xor(ga.s, o, 64); t.c = __shfl_xor(ga.c, o, 64); ksum_merge(ga, t);
+    t.s = __shfl_xor(gb.s, o, 64); t.c = __shfl_xor(gb.c, o, 64); ksum_merge(gb, t);
+    t.s = __shfl_xor(gc.s, o, 64); t.c = __shfl_xor(gc.c, o, 64); ksum_merge(gc, t);
   }
   __shared__ KSum sh[3][kEmBlock / 64];
   const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) { sh[0][w] = a; sh[1][w] = b; sh[2][w] = cc; }
+  if ((threadIdx.x & 63) == 0) { sh[0][w] = ga; sh[1][w] = gb; sh[2][w] = gc; }
   __syncthreads();
   if (threadIdx.x == 0) {
     KSum A = sh[0][0], B = sh[1][0], Cc = sh[2][0];
     for (int i = 1; i < kEmBlock / 64; i++) {
       ksum_merge(A, sh[0][i]); ksum_merge(B, sh[1][i]); ksum_merge(Cc, sh[2][i]);
     }
-    GradSlot &g = slots[blockIdx.x];
-    KSum m{g.m2_s, g.m2_c}, h{g.h_s, g.h_c}, l{g.l_s, g.l_c};
-    ksum_merge(m, A); ksum_merge(h, B); ksum_merge(l, Cc);
-    g = GradSlot{m.s, m.c, h.s, h.c, l.s, l.c};
+    GradSlot &gs = slots[blockIdx.x];
+    KSum mm{gs.m2_s, gs.m2_c}, hh{gs.h_s, gs.h_c}, ll{gs.l_s, gs.l_c};
+    ksum_merge(mm, A); ksum_merge(hh, B); ksum_merge(ll, Cc);
+    gs = GradSlot{mm.s, mm.c, hh.s, hh.c, ll.s, ll.c};
   }
 }
 
@@ -304,7 +409,7 @@ int lgv_em_create(const lgv_em_params *p, int nsteps, int ckpt_every, int device
   h->nseg = (nsteps + ckpt_every - 1) / ckpt_every;
   h->device = device;
   h->V = (int64_t)p->N * p->N * p->N;
-  h->nblocks = (int)((h->V + kEmBlock - 1) / kEmBlock);
+  h->nblocks = em_blocks(p->N);
   std::seed_seq seq = {p->seed};
   uint32_t s32 = 0;
   seq.generate(&s32, &s32 + 1);
