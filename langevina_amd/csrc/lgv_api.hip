@@ -91,7 +91,8 @@ struct lgv_ctx {
   ReplicaState *dS = nullptr;
   std::vector<void *> fields;
   Plan plan_half, plan_full, plan_ideal, plan_charge, plan_hb;
-  bool wide = false;  // some plane is >= 4 GiB: kernels use 64-bit offsets
+  bool wide = false;    // some plane is >= 4 GiB: kernels use 64-bit offsets
+  bool nt_stores = false;  // one buffer's fields exceed kStreamBytes
   MonSlot *mon_phi = nullptr, *mon_charge = nullptr, *mon_ideal = nullptr;
   EnergySlot *eslots = nullptr;
   double coef_hb = NAN, coef_charge = NAN, coef_ideal = NAN;
@@ -152,7 +153,9 @@ int build_ideal_plan(lgv_ctx *c) {
     HIPCHECK(hipFree(c->plan_ideal.d_blk_first));
     c->plan_ideal = Plan{};
   }
-  return build_plan(c->plan_ideal, work, c->wide);
+  const int rc = build_plan(c->plan_ideal, work, c->wide);
+  c->plan_ideal.stream = c->nt_stores;
+  return rc;
 }
 
 // Blocks of the phi march kernel: (x-tile, y-tile, z-chunk) per replica.
@@ -167,7 +170,9 @@ int build_hb_plan(lgv_ctx *c) {
     work.push_back((int64_t)((p.NH + TXH - 1) / TXH) * ((p.N + TY - 1) / TY) *
                    ((p.N + TZ - 1) / TZ) * kBlock);
   }
-  return build_plan(c->plan_hb, work, c->wide);
+  const int rc = build_plan(c->plan_hb, work, c->wide);
+  c->plan_hb.stream = c->nt_stores;
+  return rc;
 }
 
 // Blocks of the charge march kernel: (x-tile, line tile, chunk) per replica.
@@ -372,6 +377,7 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
   c->noise.resize(c->nrep);
   const size_t esz = c->fp32 ? sizeof(float) : sizeof(double);
   std::vector<int64_t> whalf, wfull;
+  uint64_t buffer_bytes = 0;  // one buffer: 10 planes x 2 parities
   std::vector<ReplicaState> hs(c->nrep);
   for (int r = 0; r < c->nrep; r++) {
     const lgv_model_params &m = c->mp[r];
@@ -408,10 +414,14 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
     whalf.push_back(half);
     wfull.push_back(2 * half);
     if ((uint64_t)half * esz > 0xFFFFFFFFull) c->wide = true;
+    buffer_bytes += (uint64_t)20 * half * esz;
     // test hook: LGV_FORCE_WIDE=1 runs the 64-bit-offset kernels on any size
     if (const char *fw = getenv("LGV_FORCE_WIDE")) c->wide = c->wide || atoi(fw) != 0;
     c->maxN = std::max(c->maxN, N);
   }
+  c->nt_stores = buffer_bytes > kStreamBytes;
+  // test hook: LGV_NT_STORES=0/1 forces the store mode on any size
+  if (const char *nt = getenv("LGV_NT_STORES")) c->nt_stores = atoi(nt) != 0;
   HIPCHECK(hipMalloc(&c->dP, c->nrep * sizeof(ReplicaParams)));
   HIPCHECK(hipMemcpy(c->dP, c->hp.data(), c->nrep * sizeof(ReplicaParams),
                      hipMemcpyHostToDevice));

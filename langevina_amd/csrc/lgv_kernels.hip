@@ -47,6 +47,15 @@ struct GPlane {
   __device__ __forceinline__ void st(uint32_t i, double v) const {
     *reinterpret_cast<__attribute__((address_space(1))) T *>(b + off(i)) = (T)v;
   }
+  // NTS: a non-temporal store (not kept in the caches)
+  template <bool NTS>
+  __device__ __forceinline__ void put(uint32_t i, double v) const {
+    if constexpr (NTS)
+      __builtin_nontemporal_store(
+          (T)v, reinterpret_cast<__attribute__((address_space(1))) T *>(b + off(i)));
+    else
+      st(i, v);
+  }
 };
 template <typename T, bool W>
 __device__ __forceinline__ GPlane<T, W> gplane(const ReplicaParams &p, int cur, int c, int par) {
@@ -221,7 +230,7 @@ __device__ __forceinline__ int wrapz(int z, int N) { return z < 0 ? z + N : (z >
 // grids alternate, one barrier per plane), the q values of the thread's own
 // half-column at z-1 and z+1 in registers.  Each q value is loaded once per
 // plane (plus the ring), each decision is made once.
-template <typename T, bool W>
+template <typename T, bool W, bool NTS>
 __global__ __launch_bounds__(kBlock) void k_hb_march(const ReplicaParams *__restrict__ P,
                                                      const ReplicaState *__restrict__ S,
                                                      const int *__restrict__ blk_rep,
@@ -309,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_hb_march(const ReplicaParams *__rest
       hb_decide(phi_o, heff, Lambda2, lambda, H, rdtg, gg, phi_n, cnt, dSd, dSu, true);
       const uint32_t i = idx(xh, y, z);
 #pragma unroll
-      for (int l = 0; l < 4; l++) gplane<T, W>(p, cur, l, ieo).st(i, phi_n[l]);
+      for (int l = 0; l < 4; l++) gplane<T, W>(p, cur, l, ieo).template put<NTS>(i, phi_n[l]);
 #pragma unroll
       for (int l = 0; l < 4; l++) {
         qm[l] = qc[l];
@@ -580,7 +589,7 @@ __device__ __forceinline__ void load_site(const ReplicaParams &p, int cur, int x
   for (int c = 0; c < 10; c++) f[c] = gplane<T, W>(p, cur, c, par).ld(i);
 }
 
-template <typename T, bool W>
+template <typename T, bool W, bool NTS>
 __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__restrict__ P,
                                                         const ReplicaState *__restrict__ S,
                                                         const int *__restrict__ blk_rep,
@@ -759,11 +768,11 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
         }
         rotate_site_fast(cp, A, V, h);
 #pragma unroll
-        for (int c = 0; c < 4; c++) gplane<T, W>(p, nxt, c, par).st(i, cp[c]);
+        for (int c = 0; c < 4; c++) gplane<T, W>(p, nxt, c, par).template put<NTS>(i, cp[c]);
 #pragma unroll
-        for (int c = 0; c < 3; c++) gplane<T, W>(p, nxt, 4 + c, par).st(i, A[c]);
+        for (int c = 0; c < 3; c++) gplane<T, W>(p, nxt, 4 + c, par).template put<NTS>(i, A[c]);
 #pragma unroll
-        for (int c = 0; c < 3; c++) gplane<T, W>(p, nxt, 7 + c, par).st(i, V[c]);
+        for (int c = 0; c < 3; c++) gplane<T, W>(p, nxt, 7 + c, par).template put<NTS>(i, V[c]);
       }
     }
     __syncthreads();
@@ -1081,6 +1090,28 @@ void launch_wall_dft(int N, const double *wall, const double *wallphase, double 
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 // W: some plane of the launch is 4 GiB or larger (64-bit element offsets).
+// The same over the store mode as well (plan.stream).
+#define LGV_DISPATCH_S(KERNEL, GRID, ST, ...)                                             \
+  do {                                                                                    \
+    if (plan.stream) {                                                                    \
+      if (fp32) {                                                                         \
+        if (plan.wide) KERNEL<float, true, true><<<GRID, kBlock, 0, ST>>>(__VA_ARGS__);   \
+        else KERNEL<float, false, true><<<GRID, kBlock, 0, ST>>>(__VA_ARGS__);            \
+      } else {                                                                            \
+        if (plan.wide) KERNEL<double, true, true><<<GRID, kBlock, 0, ST>>>(__VA_ARGS__);  \
+        else KERNEL<double, false, true><<<GRID, kBlock, 0, ST>>>(__VA_ARGS__);           \
+      }                                                                                   \
+    } else {                                                                              \
+      if (fp32) {                                                                         \
+        if (plan.wide) KERNEL<float, true, false><<<GRID, kBlock, 0, ST>>>(__VA_ARGS__);  \
+        else KERNEL<float, false, false><<<GRID, kBlock, 0, ST>>>(__VA_ARGS__);           \
+      } else {                                                                            \
+        if (plan.wide) KERNEL<double, true, false><<<GRID, kBlock, 0, ST>>>(__VA_ARGS__); \
+        else KERNEL<double, false, false><<<GRID, kBlock, 0, ST>>>(__VA_ARGS__);          \
+      }                                                                                   \
+    }                                                                                     \
+  } while (0)
+
 #define LGV_DISPATCH(KERNEL, GRID, ST, ...)                                           \
   do {                                                                                \
     if (fp32) {                                                                       \
@@ -1100,7 +1131,7 @@ void launch_hb_phi(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaS
 
 void launch_hb_march(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
                      int ieo, int ctr_off, MonSlot *mon, hipStream_t st) {
-  LGV_DISPATCH(k_hb_march, plan.nblocks, st, P, S, plan.d_blk_rep, plan.d_blk_first, ieo,
+  LGV_DISPATCH_S(k_hb_march, plan.nblocks, st, P, S, plan.d_blk_rep, plan.d_blk_first, ieo,
                ctr_off, mon);
 }
 
@@ -1118,7 +1149,7 @@ void launch_charge_march(bool fp32, const Plan &plan, const ReplicaParams *P, Re
 
 void launch_ideal(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
                   int nrep, hipStream_t st) {
-  LGV_DISPATCH(k_ideal_fused, plan.nblocks, st, P, S, plan.d_blk_rep, plan.d_blk_first);
+  LGV_DISPATCH_S(k_ideal_fused, plan.nblocks, st, P, S, plan.d_blk_rep, plan.d_blk_first);
   k_flip<<<nblk(nrep, 64), 64, 0, st>>>(S, nrep);
 }
 

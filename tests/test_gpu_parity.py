@@ -327,6 +327,26 @@ def test_full_step_fp32(L, orc):
     ctx.close()
 
 
+def test_nontemporal_store_kernels_match(L, monkeypatch):
+    """The non-temporal-store instantiations of the phi march and the ideal
+    kernel (chosen for fields above 1 GiB) are forced on a small lattice and
+    give the bits of the plain-store ones, in both storage precisions."""
+    log("nontemporal stores")
+    N = 32
+    u = np.random.default_rng(23).normal(size=(N, N, N, 10)) * 0.6
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("LGV_NT_STORES", mode)
+        for prec in (0, 1):
+            ctx = L.Context(L.model_params(N, seed=4, **PAR), precision=prec)
+            ctx.set_field(u.astype(np.float32).astype(np.float64) if prec else u)
+            ctx.run(0.04, 2)
+            out[mode, prec] = ctx.get_field()
+            ctx.close()
+    for prec in (0, 1):
+        np.testing.assert_array_equal(out["0", prec], out["1", prec])
+
+
 def test_64bit_offset_kernels_match(L, orc, monkeypatch):
     """The 64-bit-offset instantiation (planes of 4 GiB or more) is forced on a
     small lattice and must give the same bits as the 32-bit one."""
