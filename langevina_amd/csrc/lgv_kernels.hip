@@ -47,6 +47,15 @@ struct GPlane {
   __device__ __forceinline__ void st(uint32_t i, double v) const {
     *reinterpret_cast<__attribute__((address_space(1))) T *>(b + off(i)) = (T)v;
   }
+  // NTS: a non-temporal load, for a value no other thread reads
+  template <bool NTS>
+  __device__ __forceinline__ double get(uint32_t i) const {
+    if constexpr (NTS)
+      return (double)__builtin_nontemporal_load(
+          reinterpret_cast<const __attribute__((address_space(1))) T *>(b + off(i)));
+    else
+      return ld(i);
+  }
   // NTS: a non-temporal store (not kept in the caches)
   template <bool NTS>
   __device__ __forceinline__ void put(uint32_t i, double v) const {
@@ -294,7 +303,10 @@ __global__ __launch_bounds__(kBlock) void k_hb_march(const ReplicaParams *__rest
     double phi_o[4];
     if (own) {
       ld4(q, idx(xh, y, zp), qp);
-      ld4(ieo, idx(xh, y, z), phi_o);
+      // the update parity's own value: read by this thread only
+#pragma unroll
+      for (int l = 0; l < 4; l++)
+        phi_o[l] = gplane<T, W>(p, cur, l, ieo).template get<NTS>(idx(xh, y, z));
       const int pos = (ty + 1) * GW + (tx + 1);
 #pragma unroll
       for (int l = 0; l < 4; l++) g[l][pos] = (T)qc[l];
