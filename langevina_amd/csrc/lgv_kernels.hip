@@ -464,7 +464,7 @@ __device__ __forceinline__ void charge_line(const ReplicaParams &p, int cur, int
 
 // The same two sub-sweeps with every site loaded once and every pair
 // computed once (except one first-parity pair per chunk end).  Blocks tile
-// the lattice as TXH half-columns x TQ lines x a chunk of kChargeChunk sites:
+// the lattice as TXH half-columns x TQ lines x a chunk of C sites (charge_tile):
 //  * d = y or z: a thread owns the two columns x = 2xh, 2xh+1 along d and
 //    marches d through its chunk.  Along a column the parity alternates; at
 //    each step it updates the next first-parity pair (m, m+1) and then the
