@@ -50,11 +50,19 @@ MEASURED_TRAFFIC = {
 }
 
 
-def tables(fp32):
+# The library runs the phi z-march for FP64 fields above 1 GiB (one buffer,
+# all replicas) and the one-thread-per-site phi kernel otherwise.
+LARGE_FIELD_BYTES = 2 ** 30
+
+
+def tables(fp32, field_bytes=None):
     """(algorithmic bytes per site per launch, kernel name, schedule bytes per
-    site-update) for the precision's kernel schedule."""
+    site-update) for the precision's kernel schedule; field_bytes (one
+    buffer) decides the phi kernel."""
     if fp32:
         return ALG_BYTES_FP32, KERNEL_FP32, SCHEDULE_BYTES_PER_SITE_UPDATE_FP32
+    if field_bytes is not None and field_bytes <= LARGE_FIELD_BYTES:
+        return ALG_BYTES, dict(KERNEL, phi="k_hb_phi"), SCHEDULE_BYTES_PER_SITE_UPDATE
     return ALG_BYTES, KERNEL, SCHEDULE_BYTES_PER_SITE_UPDATE
 
 
@@ -242,7 +250,7 @@ def bench_batched(args):
         c1.close()
     seq_value = sites * args.steps / seq_el
     esz = 8 if prec == L.FP64 else 4
-    alg, kern, _ = tables(prec == L.FP32)
+    alg, kern, _ = tables(prec == L.FP32, sites * 10 * esz)
     dom = max(("ideal", "phi", "charge"), key=lambda k: tm[k][0])
     ms, launches = tm[dom]
     per_launch_s = ms * 1e-3 / max(launches, 1)
@@ -397,7 +405,7 @@ def main():
     value = sites * args.steps / el
     ms_per_step = el * 1e3 / args.steps
     # dominant kernel class: the one with the largest total time
-    alg, kern, sched = tables(prec == L.FP32)
+    alg, kern, sched = tables(prec == L.FP32, N ** 3 * 10 * esz)
     dom = max(("ideal", "phi", "charge"), key=lambda k: tm[k][0])
     ms, launches = tm[dom]
     per_launch_s = ms * 1e-3 / max(launches, 1)

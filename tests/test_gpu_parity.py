@@ -328,9 +328,10 @@ def test_full_step_fp32(L, orc):
 
 
 def test_nontemporal_store_kernels_match(L, monkeypatch):
-    """The non-temporal-store instantiations of the phi march and the ideal
-    kernel (chosen for fields above 1 GiB) are forced on a small lattice and
-    give the bits of the plain-store ones, in both storage precisions."""
+    """The large-field mode (chosen for fields above 1 GiB: the FP64 phi march
+    and non-temporal stores in it and in the ideal kernel) is forced on a small
+    lattice and gives the bits of the default mode (per-site phi kernel, plain
+    stores), in both storage precisions."""
     log("nontemporal stores")
     N = 32
     u = np.random.default_rng(23).normal(size=(N, N, N, 10)) * 0.6
@@ -353,14 +354,16 @@ def test_64bit_offset_kernels_match(L, orc, monkeypatch):
     log("wide offsets")
     N = 16
     u = np.random.default_rng(17).normal(size=(N, N, N, 10)) * 0.6
-    out = []
-    for wide in ("0", "1"):
-        monkeypatch.setenv("LGV_FORCE_WIDE", wide)
-        for prec in (0, 1):
-            ctx = L.Context(L.model_params(N, seed=3, **PAR), precision=prec)
-            ctx.set_field(u.astype(np.float32).astype(np.float64) if prec else u)
-            ctx.run(0.04, 2)
-            out.append(ctx.get_field())
-            ctx.close()
-    np.testing.assert_array_equal(out[0], out[2])
-    np.testing.assert_array_equal(out[1], out[3])
+    out = {}
+    for nt in ("0", "1"):   # both phi kernels and store modes
+        monkeypatch.setenv("LGV_NT_STORES", nt)
+        for wide in ("0", "1"):
+            monkeypatch.setenv("LGV_FORCE_WIDE", wide)
+            for prec in (0, 1):
+                ctx = L.Context(L.model_params(N, seed=3, **PAR), precision=prec)
+                ctx.set_field(u.astype(np.float32).astype(np.float64) if prec else u)
+                ctx.run(0.04, 2)
+                out[nt, wide, prec] = ctx.get_field()
+                ctx.close()
+    for key, val in out.items():
+        np.testing.assert_array_equal(val, out["0", "0", key[2]], err_msg=str(key))
