@@ -98,8 +98,10 @@ struct lgv_ctx {
   double coef_hb = NAN, coef_charge = NAN, coef_ideal = NAN;
   double *d_aos = nullptr;
   int maxN = 0;
-  double *d_partXZ = nullptr, *d_partXY = nullptr, *d_wall = nullptr, *d_wallphase = nullptr;
-  double *d_wallk = nullptr, *d_wallphasek = nullptr;
+  double *d_partXZ = nullptr, *d_partXY = nullptr;
+  // walls, phase walls and their transforms of one measurement, packed for
+  // the current N (see MeasBuf) so that one copy brings them to the host
+  double *d_meas = nullptr;
   bool use_graph = false;
   hipGraphExec_t graph_exec = nullptr;
   double graph_dt = NAN;
@@ -113,6 +115,22 @@ struct lgv_ctx {
 };
 
 namespace {
+
+// Layout of one measurement in a packed buffer for side N: walls [3][11][N],
+// phase walls [3][14][N], their transforms [3][11][nk][2] and [3][14][nk][2].
+struct MeasBuf {
+  double *wall, *wallphase, *wallk, *wallphasek;
+  int64_t total;  // doubles
+  static int64_t size(int N) { return 3 * 25 * (int64_t)N + 3 * 25 * (int64_t)(N / 2 + 1) * 2; }
+  MeasBuf(double *base, int N) {
+    const int64_t nk = N / 2 + 1;
+    wall = base;
+    wallphase = wall + 3 * 11 * (int64_t)N;
+    wallk = wallphase + 3 * 14 * (int64_t)N;
+    wallphasek = wallk + 3 * 11 * nk * 2;
+    total = size(N);
+  }
+};
 
 int build_plan(Plan &plan, const std::vector<int64_t> &work, bool wide) {
   plan.wide = wide;
@@ -449,12 +467,10 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
   HIPCHECK(hipMalloc(&c->eslots, c->plan_full.nblocks * sizeof(EnergySlot)));
   const int64_t maxV = (int64_t)c->maxN * c->maxN * c->maxN;
   HIPCHECK(hipMalloc(&c->d_aos, maxV * 10 * sizeof(double)));
-  HIPCHECK(hipMalloc(&c->d_partXZ, (int64_t)25 * c->maxN * c->maxN * sizeof(double)));
-  HIPCHECK(hipMalloc(&c->d_partXY, (int64_t)25 * c->maxN * c->maxN * sizeof(double)));
-  HIPCHECK(hipMalloc(&c->d_wall, 3 * 11 * c->maxN * sizeof(double)));
-  HIPCHECK(hipMalloc(&c->d_wallphase, 3 * 14 * c->maxN * sizeof(double)));
-  HIPCHECK(hipMalloc(&c->d_wallk, 3 * 11 * (c->maxN / 2 + 1) * 2 * sizeof(double)));
-  HIPCHECK(hipMalloc(&c->d_wallphasek, 3 * 14 * (c->maxN / 2 + 1) * 2 * sizeof(double)));
+  const int64_t part = 25 * std::max(kMeasureColumns, (int64_t)c->maxN * c->maxN);
+  HIPCHECK(hipMalloc(&c->d_partXZ, part * sizeof(double)));
+  HIPCHECK(hipMalloc(&c->d_partXY, part * sizeof(double)));
+  HIPCHECK(hipMalloc(&c->d_meas, MeasBuf::size(c->maxN) * sizeof(double)));
   HIPCHECK(hipDeviceSynchronize());
   *out = c.release();
   return LGV_OK;
@@ -480,8 +496,7 @@ int lgv_destroy(lgv_ctx *c) {
   }
   for (void *q : {(void *)c->mon_phi, (void *)c->mon_charge, (void *)c->mon_ideal,
                   (void *)c->eslots, (void *)c->d_aos, (void *)c->d_partXZ, (void *)c->d_partXY,
-                  (void *)c->d_wall, (void *)c->d_wallphase, (void *)c->d_wallk,
-                  (void *)c->d_wallphasek})
+                  (void *)c->d_meas})
     note(hipFree(q));
   if (c->stream) note(hipStreamDestroy(c->stream));
   delete c;
@@ -728,13 +743,16 @@ int lgv_measure(lgv_ctx *c, int r, double *wall, double *wallphase, double *oavg
   if (!wall || !wallphase || !oavg) return fail(LGV_ERR_ARG, "null output");
   HIPCHECK(hipSetDevice(c->device));
   const int N = c->hp[r].N;
-  launch_measure(c->fp32, c->dP, c->dS, r, N, c->d_partXZ, c->d_partXY, c->d_wall, c->d_wallphase,
+  const MeasBuf mb(c->d_meas, N);
+  launch_measure(c->fp32, c->dP, c->dS, r, N, c->d_partXZ, c->d_partXY, mb.wall, mb.wallphase,
                  c->stream);
   if ((rc = check_launch())) return rc;
+  std::vector<double> h(3 * 25 * (size_t)N);  // walls and phase walls, one copy
+  HIPCHECK(hipMemcpyAsync(h.data(), mb.wall, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
-  HIPCHECK(hipMemcpy(wall, c->d_wall, 3 * 11 * N * sizeof(double), hipMemcpyDeviceToHost));
-  HIPCHECK(hipMemcpy(wallphase, c->d_wallphase, 3 * 14 * N * sizeof(double),
-                     hipMemcpyDeviceToHost));
+  memcpy(wall, h.data(), 3 * 11 * N * sizeof(double));
+  memcpy(wallphase, h.data() + 3 * 11 * N, 3 * 14 * N * sizeof(double));
   // computeDerivedObs's OAverage (measurer.cxx:288-300): mean of wallX rows
   for (int l = 0; l < 11; l++) {
     double s = 0.;
@@ -824,18 +842,20 @@ int lgv_measure_full(lgv_ctx *c, int r, lgv_measurement *m) {
   HIPCHECK(hipSetDevice(c->device));
   const int N = c->hp[r].N, nk = N / 2 + 1;
   std::vector<double> wall(3 * 11 * N), wallphase(3 * 14 * N), oavg(11);
-  launch_measure(c->fp32, c->dP, c->dS, r, N, c->d_partXZ, c->d_partXY, c->d_wall,
-                 c->d_wallphase, c->stream);
-  launch_wall_dft(N, c->d_wall, c->d_wallphase, c->d_wallk, c->d_wallphasek, c->stream);
+  const MeasBuf mb(c->d_meas, N);
+  launch_measure(c->fp32, c->dP, c->dS, r, N, c->d_partXZ, c->d_partXY, mb.wall, mb.wallphase,
+                 c->stream);
+  launch_wall_dft(N, mb.wall, mb.wallphase, mb.wallk, mb.wallphasek, c->stream);
   if ((rc = check_launch())) return rc;
+  std::vector<double> h(mb.total);  // everything in one copy
+  HIPCHECK(hipMemcpyAsync(h.data(), mb.wall, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
-  HIPCHECK(hipMemcpy(wall.data(), c->d_wall, wall.size() * sizeof(double), hipMemcpyDeviceToHost));
-  HIPCHECK(hipMemcpy(wallphase.data(), c->d_wallphase, wallphase.size() * sizeof(double),
-                     hipMemcpyDeviceToHost));
   std::vector<double> wk(3 * 11 * nk * 2), wpk(3 * 14 * nk * 2);
-  HIPCHECK(hipMemcpy(wk.data(), c->d_wallk, wk.size() * sizeof(double), hipMemcpyDeviceToHost));
-  HIPCHECK(hipMemcpy(wpk.data(), c->d_wallphasek, wpk.size() * sizeof(double),
-                     hipMemcpyDeviceToHost));
+  memcpy(wall.data(), h.data(), wall.size() * sizeof(double));
+  memcpy(wallphase.data(), h.data() + (mb.wallphase - mb.wall), wallphase.size() * sizeof(double));
+  memcpy(wk.data(), h.data() + (mb.wallk - mb.wall), wk.size() * sizeof(double));
+  memcpy(wpk.data(), h.data() + (mb.wallphasek - mb.wall), wpk.size() * sizeof(double));
   for (int l = 0; l < 11; l++) {
     double s = 0.;
     for (int i = 0; i < N; i++) s += wall[l * N + i];

@@ -120,6 +120,17 @@ constexpr uint64_t kStreamBytes = 1ull << 30;
 // Largest side: N^3 must fit 32-bit site indices.
 constexpr int kMaxN = 1536;
 
+// Chunks of the measurement's column sums: about 2^19 column-chunks per
+// pass (8 chunks at N = 256, 1 from N = 724 on).  The partial buffers hold
+// 25 * measure_chunks(N) * N^2 <= 25 * max(2^19, N^2) doubles.
+constexpr int64_t kMeasureColumns = int64_t(1) << 19;
+__host__ __device__ inline int measure_chunks(int N) {
+  const int64_t n2 = (int64_t)N * N;
+  int c = 1;
+  while (c < 8 && 2 * c * n2 <= kMeasureColumns) c *= 2;
+  return c;
+}
+
 // Kernel launchers (lgv_kernels.hip).  fp32 selects the storage type.
 void launch_hb_phi(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
                    int ieo, int ctr_off, MonSlot *mon, hipStream_t st);

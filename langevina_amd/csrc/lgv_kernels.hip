@@ -986,14 +986,14 @@ __device__ __forceinline__ void contract_rho(const double n[4], const double rho
 }
 
 template <typename T>
-__device__ __forceinline__ void site_obs(const ReplicaParams &p, int cur, int par, int64_t i,
-                                         double o[25]) {
+__device__ __forceinline__ void site_obs_add(const ReplicaParams &p, int cur, int par, int64_t i,
+                                             KSum acc[25]) {
   double f[10];
   for (int c = 0; c < 10; c++) f[c] = (double)plane<T>(p, cur, c, par)[i];
   double phi2 = 0.0;
   for (int l = 0; l < 4; l++) phi2 += f[l] * f[l];
-  for (int c = 0; c < 10; c++) o[c] = f[c];
-  o[10] = phi2;
+  for (int c = 0; c < 10; c++) ksum_add(acc[c], f[c]);
+  ksum_add(acc[10], phi2);
   const double sigma = sqrt(phi2);
   double n[4];
   if (phi2 > 100. * DBL_MIN) {
@@ -1006,69 +1006,118 @@ __device__ __forceinline__ void site_obs(const ReplicaParams &p, int cur, int pa
   double A[4], V[4];
   contract_rho(n, rho, A);
   contract_rho(n, drho, V);
-  o[11] = sigma;
+  ksum_add(acc[11], sigma);
   for (int l = 0; l < 4; l++) {
-    o[12 + l] = n[l];
-    o[16 + l] = A[l];
-    o[20 + l] = V[l];
+    ksum_add(acc[12 + l], n[l]);
+    ksum_add(acc[16 + l], A[l]);
+    ksum_add(acc[20 + l], V[l]);
   }
-  o[24] = sigma * sigma;
+  ksum_add(acc[24], sigma * sigma);
 }
 
-// One thread per (x, z): compensated sum over y -> partXZ[o][z][x];
-// one thread per (x, y): compensated sum over z -> partXY[o][y][x].
+// Per (x, z): compensated sum over y of the 25 site observables ->
+// partXZ[ch][o][z][x]; per (x, y): sum over z -> partXY[ch][o][y][x].  The
+// summed axis is split into nch chunks (measure_chunks) so that a 256^3
+// lattice runs 8 N^2 threads instead of N^2; k_measure_fold adds the chunks.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_measure_cols(const ReplicaParams *P,
                                                          const ReplicaState *S, int replica,
-                                                         int along_z, double *part) {
-  const ReplicaParams p = P[replica];
+                                                         int along_z, int nch, double *part) {
+  const ReplicaParams &p = P[replica];
   const int cur = live(S[replica]);
   const int N = p.N;
+  const int64_t N2 = (int64_t)N * N;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)N * N) return;
-  const int x = (int)(t % N), w = (int)(t / N);
+  if (t >= N2 * nch) return;
+  const int ch = (int)(t / N2);
+  const int64_t col = t - (int64_t)ch * N2;
+  const int x = (int)(col % N), w = (int)(col / N);
+  const int v0 = (int)((int64_t)ch * N / nch), v1 = (int)((int64_t)(ch + 1) * N / nch);
   KSum acc[25];
   for (int o = 0; o < 25; o++) acc[o] = KSum{0., 0.};
-  for (int v = 0; v < N; v++) {
+  for (int v = v0; v < v1; v++) {
     const int y = along_z ? w : v;
     const int z = along_z ? v : w;
     const int par = (x + y + z) & 1;
     const int64_t i = ((int64_t)z * N + y) * p.NH + (x >> 1);
-    double o[25];
-    site_obs<T>(p, cur, par, i, o);
-    for (int k = 0; k < 25; k++) ksum_add(acc[k], o[k]);
+    site_obs_add<T>(p, cur, par, i, acc);
   }
-  for (int k = 0; k < 25; k++) part[((int64_t)k * N + w) * N + x] = ksum_value(acc[k]);
+  double *out = part + (int64_t)ch * 25 * N2;
+  for (int k = 0; k < 25; k++) out[((int64_t)k * N + w) * N + x] = ksum_value(acc[k]);
 }
 
-// wall[d][obs][n]: d = 0 (x) sums partXZ over z, d = 2 (z) sums partXZ over
-// x, d = 1 (y) sums partXY over x; then / N^2 as measurer.cxx:274-280.
-__global__ void k_measure_reduce(int N, const double *partXZ, const double *partXY,
-                                 double *wall, double *wallphase) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 3 * 25 * N) return;
-  const int n = t % N;
-  const int k = (t / N) % 25;
-  const int d = t / (25 * N);
+// Folds the chunk partials into chunk 0: part[0][j] = compensated sum over
+// ch of part[ch][j], j < 25 N^2 (one thread per element, coalesced).
+__global__ void k_measure_fold(int64_t n, int nch, double *part) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
   KSum acc{0., 0.};
-  for (int m = 0; m < N; m++) {
-    double v;
-    if (d == 0) v = partXZ[((int64_t)k * N + m) * N + n];
-    else if (d == 2) v = partXZ[((int64_t)k * N + n) * N + m];
-    else v = partXY[((int64_t)k * N + n) * N + m];
-    ksum_add(acc, v);
+  for (int ch = 0; ch < nch; ch++) ksum_add(acc, part[(int64_t)ch * n + j]);
+  part[j] = ksum_value(acc);
+}
+
+// Walls, x direction: wall_x[k][n] = (compensated sum over z of
+// partXZ[k][z][n]) / N^2.  A block takes 64 consecutive n (coalesced) of one
+// k; its 8 thread groups sum z = g, g + 8, ... and the 8 partials are merged
+// in group order.
+__global__ __launch_bounds__(512) void k_measure_reduce_x(int N, const double *partXZ,
+                                                          double *wall, double *wallphase) {
+  __shared__ KSum part[8][64];
+  const int nb = (N + 63) / 64;
+  const int k = blockIdx.x / nb;
+  const int n = (blockIdx.x % nb) * 64 + (threadIdx.x & 63);
+  const int g = threadIdx.x >> 6;
+  KSum acc{0., 0.};
+  if (n < N)
+    for (int m = g; m < N; m += 8) ksum_add(acc, partXZ[((int64_t)k * N + m) * N + n]);
+  part[g][threadIdx.x & 63] = acc;
+  __syncthreads();
+  if (g == 0 && n < N) {
+    for (int i = 1; i < 8; i++) ksum_merge(acc, part[i][threadIdx.x]);
+    const double val = ksum_value(acc) / (double)(N * N);
+    if (k < 11) wall[(int64_t)k * N + n] = val;
+    else wallphase[(int64_t)(k - 11) * N + n] = val;
   }
-  const double val = ksum_value(acc) / (double)(N * N);
-  if (k < 11) wall[((int64_t)d * 11 + k) * N + n] = val;
-  else wallphase[((int64_t)d * 14 + (k - 11)) * N + n] = val;
+}
+
+// Walls, y and z directions: row sums, one wave per (d, k, n) row (lanes
+// read consecutive x; lane partials merged by a fixed shuffle tree).
+//   d = 1: wall_y[k][n] = sum_x partXY[k][n][x] / N^2
+//   d = 2: wall_z[k][n] = sum_x partXZ[k][n][x] / N^2
+__global__ __launch_bounds__(256) void k_measure_reduce_rows(int N, const double *partXZ,
+                                                             const double *partXY,
+                                                             double *wall, double *wallphase) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= 2 * 25 * N) return;  // whole waves exit together
+  const int n = row % N, k = (row / N) % 25, d = 1 + row / (25 * N);
+  const double *src = (d == 1 ? partXY : partXZ) + ((int64_t)k * N + n) * N;
+  KSum acc{0., 0.};
+  for (int x = lane; x < N; x += 64) ksum_add(acc, src[x]);
+  for (int o = 32; o > 0; o >>= 1) {
+    KSum t;
+    t.s = __shfl_xor(acc.s, o, 64);
+    t.c = __shfl_xor(acc.c, o, 64);
+    ksum_merge(acc, t);
+  }
+  if (lane == 0) {
+    const double val = ksum_value(acc) / (double)(N * N);
+    if (k < 11) wall[((int64_t)d * 11 + k) * N + n] = val;
+    else wallphase[((int64_t)d * 14 + (k - 11)) * N + n] = val;
+  }
 }
 
 // measurer_fft::execute_row (measurer.h:33-41): FFTW's r2c forward
 // transform divided by N, out[k] = (1/N) sum_x in[x] e^{-2 pi i k x / N},
 // k = 0..N/2, as a direct DFT with exactly reduced twiddles and compensated
 // sums.  One thread per (row, k); rows = 3 x 25 walls.
-__global__ void k_wall_dft(int N, const double *wall, const double *wallphase, double *wall_k,
-                           double *wallphase_k) {
+__global__ __launch_bounds__(256) void k_wall_dft(int N, const double *wall,
+                                                  const double *wallphase, double *wall_k,
+                                                  double *wallphase_k) {
+  // twiddles cos/sin(2 pi m / N), m = 0 .. N-1, once per block
+  __shared__ double tc[kMaxN], ts[kMaxN];
+  for (int m = threadIdx.x; m < N; m += blockDim.x) sincospi(2.0 * m / N, &ts[m], &tc[m]);
+  __syncthreads();
   const int nk = N / 2 + 1;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 3 * 25 * nk) return;
@@ -1078,12 +1127,12 @@ __global__ void k_wall_dft(int N, const double *wall, const double *wallphase, d
   const double *in = o < 11 ? wall + ((int64_t)d * 11 + o) * N
                             : wallphase + ((int64_t)d * 14 + (o - 11)) * N;
   KSum re{0., 0.}, im{0., 0.};
+  int m = 0;  // k x mod N
   for (int x = 0; x < N; x++) {
-    const int m = (int)(((int64_t)k * x) % N);
-    double s, c;
-    sincospi(2.0 * m / N, &s, &c);
-    ksum_add(re, in[x] * c);
-    ksum_add(im, -(in[x] * s));
+    ksum_add(re, in[x] * tc[m]);
+    ksum_add(im, -(in[x] * ts[m]));
+    m += k;
+    if (m >= N) m -= N;
   }
   double *out = o < 11 ? wall_k + (((int64_t)d * 11 + o) * nk + k) * 2
                        : wallphase_k + (((int64_t)d * 14 + (o - 11)) * nk + k) * 2;
@@ -1096,7 +1145,7 @@ __global__ void k_wall_dft(int N, const double *wall, const double *wallphase, d
 void launch_wall_dft(int N, const double *wall, const double *wallphase, double *wall_k,
                      double *wallphase_k, hipStream_t st) {
   const int n = 3 * 25 * (N / 2 + 1);
-  k_wall_dft<<<(n + 127) / 128, 128, 0, st>>>(N, wall, wallphase, wall_k, wallphase_k);
+  k_wall_dft<<<(n + 255) / 256, 256, 0, st>>>(N, wall, wallphase, wall_k, wallphase_k);
 }
 
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -1208,15 +1257,23 @@ void launch_rb_to_aos(bool fp32, const ReplicaParams *P, const ReplicaState *S, 
 void launch_measure(bool fp32, const ReplicaParams *P, const ReplicaState *S, int replica,
                     int N, double *partXZ, double *partXY, double *wall, double *wallphase,
                     hipStream_t st) {
-  const int64_t cols = (int64_t)N * N;
+  const int nch = measure_chunks(N);
+  const int64_t threads = (int64_t)N * N * nch;
   if (fp32) {
-    k_measure_cols<float><<<nblk(cols, kBlock), kBlock, 0, st>>>(P, S, replica, 0, partXZ);
-    k_measure_cols<float><<<nblk(cols, kBlock), kBlock, 0, st>>>(P, S, replica, 1, partXY);
+    k_measure_cols<float><<<nblk(threads, kBlock), kBlock, 0, st>>>(P, S, replica, 0, nch, partXZ);
+    k_measure_cols<float><<<nblk(threads, kBlock), kBlock, 0, st>>>(P, S, replica, 1, nch, partXY);
   } else {
-    k_measure_cols<double><<<nblk(cols, kBlock), kBlock, 0, st>>>(P, S, replica, 0, partXZ);
-    k_measure_cols<double><<<nblk(cols, kBlock), kBlock, 0, st>>>(P, S, replica, 1, partXY);
+    k_measure_cols<double><<<nblk(threads, kBlock), kBlock, 0, st>>>(P, S, replica, 0, nch, partXZ);
+    k_measure_cols<double><<<nblk(threads, kBlock), kBlock, 0, st>>>(P, S, replica, 1, nch, partXY);
   }
-  k_measure_reduce<<<nblk(3 * 25 * N, 256), 256, 0, st>>>(N, partXZ, partXY, wall, wallphase);
+  if (nch > 1) {
+    const int64_t n = (int64_t)25 * N * N;
+    k_measure_fold<<<nblk(n, 256), 256, 0, st>>>(n, nch, partXZ);
+    k_measure_fold<<<nblk(n, 256), 256, 0, st>>>(n, nch, partXY);
+  }
+  k_measure_reduce_x<<<25 * ((N + 63) / 64), 512, 0, st>>>(N, partXZ, wall, wallphase);
+  k_measure_reduce_rows<<<nblk(2 * 25 * N, 4), 256, 0, st>>>(N, partXZ, partXY, wall,
+                                                             wallphase);
 }
 
 }  // namespace lgv
