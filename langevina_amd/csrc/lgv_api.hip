@@ -116,6 +116,16 @@ struct lgv_ctx {
 
 namespace {
 
+// z-planes per staging slab; LGV_STAGE_PLANES (a test hook) lowers it.
+int slab_planes(int N) {
+  int n = stage_planes(N);
+  if (const char *e = getenv("LGV_STAGE_PLANES")) {
+    const int v = atoi(e);
+    if (v >= 1 && v < n) n = v;
+  }
+  return n;
+}
+
 // Layout of one measurement in a packed buffer for side N: walls [3][11][N],
 // phase walls [3][14][N], their transforms [3][11][nk][2] and [3][14][nk][2].
 struct MeasBuf {
@@ -386,7 +396,14 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
       return fail(LGV_ERR_ARG, "replica %d: NX=%d must be even and in [4, %d]", r, m.NX, kMaxN);
     if (!(m.chi > 0)) return fail(LGV_ERR_ARG, "replica %d: chi must be positive", r);
   }
-  std::unique_ptr<lgv_ctx> c(new lgv_ctx);
+  // on any failure below, everything allocated so far is released (the
+  // error message of the failure is kept)
+  auto cleanup = [](lgv_ctx *p) {
+    const std::string msg = g_err;
+    lgv_destroy(p);
+    g_err = msg;
+  };
+  std::unique_ptr<lgv_ctx, decltype(cleanup)> c(new lgv_ctx, cleanup);
   c->device = cfg->device;
   HIPCHECK(hipSetDevice(c->device));
   HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -465,8 +482,8 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
   HIPCHECK(hipMalloc(&c->mon_ideal, c->nrep * sizeof(MonSlot)));
   HIPCHECK(hipMemset(c->mon_ideal, 0, c->nrep * sizeof(MonSlot)));
   HIPCHECK(hipMalloc(&c->eslots, c->plan_full.nblocks * sizeof(EnergySlot)));
-  const int64_t maxV = (int64_t)c->maxN * c->maxN * c->maxN;
-  HIPCHECK(hipMalloc(&c->d_aos, maxV * 10 * sizeof(double)));
+  HIPCHECK(hipMalloc(&c->d_aos, (int64_t)stage_planes(c->maxN) * c->maxN * c->maxN * 10 *
+                                     sizeof(double)));
   const int64_t part = 25 * std::max(kMeasureColumns, (int64_t)c->maxN * c->maxN);
   HIPCHECK(hipMalloc(&c->d_partXZ, part * sizeof(double)));
   HIPCHECK(hipMalloc(&c->d_partXY, part * sizeof(double)));
@@ -511,12 +528,16 @@ int lgv_set_field(lgv_ctx *c, int r, const double *aos) {
   if (!aos) return fail(LGV_ERR_ARG, "null field");
   HIPCHECK(hipSetDevice(c->device));
   const int N = c->hp[r].N;
-  const int64_t V = (int64_t)N * N * N;
+  const int64_t P = (int64_t)N * N * 10;  // doubles per z-plane
+  const int slab = slab_planes(N);
   HIPCHECK(hipStreamSynchronize(c->stream));
-  HIPCHECK(hipMemcpy(c->d_aos, aos, V * 10 * sizeof(double), hipMemcpyHostToDevice));
-  launch_aos_to_rb(c->fp32, c->dP, c->dS, r, N, c->d_aos, c->stream);
-  if ((rc = check_launch())) return rc;
-  HIPCHECK(hipStreamSynchronize(c->stream));
+  for (int z0 = 0; z0 < N; z0 += slab) {
+    const int nz = std::min(slab, N - z0);
+    HIPCHECK(hipMemcpy(c->d_aos, aos + z0 * P, nz * P * sizeof(double), hipMemcpyHostToDevice));
+    launch_aos_to_rb(c->fp32, c->dP, c->dS, r, N, z0, nz, c->d_aos, c->stream);
+    if ((rc = check_launch())) return rc;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  }
   return LGV_OK;
 }
 
@@ -526,11 +547,15 @@ int lgv_get_field(lgv_ctx *c, int r, double *aos) {
   if (!aos) return fail(LGV_ERR_ARG, "null field");
   HIPCHECK(hipSetDevice(c->device));
   const int N = c->hp[r].N;
-  const int64_t V = (int64_t)N * N * N;
-  launch_rb_to_aos(c->fp32, c->dP, c->dS, r, N, c->d_aos, c->stream);
-  if ((rc = check_launch())) return rc;
-  HIPCHECK(hipStreamSynchronize(c->stream));
-  HIPCHECK(hipMemcpy(aos, c->d_aos, V * 10 * sizeof(double), hipMemcpyDeviceToHost));
+  const int64_t P = (int64_t)N * N * 10;  // doubles per z-plane
+  const int slab = slab_planes(N);
+  for (int z0 = 0; z0 < N; z0 += slab) {
+    const int nz = std::min(slab, N - z0);
+    launch_rb_to_aos(c->fp32, c->dP, c->dS, r, N, z0, nz, c->d_aos, c->stream);
+    if ((rc = check_launch())) return rc;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    HIPCHECK(hipMemcpy(aos + z0 * P, c->d_aos, nz * P * sizeof(double), hipMemcpyDeviceToHost));
+  }
   return LGV_OK;
 }
 

@@ -150,10 +150,18 @@ void launch_energy_finish(int nrep, const Plan &plan, const ReplicaParams *P,
 void launch_advance(int nrep, ReplicaState *S, double dt, int advance_time,
                     uint64_t nctr, hipStream_t st);
 void launch_set_time(int nrep, int replica, ReplicaState *S, double t, hipStream_t st);
+// Field transfers go through a staging buffer of at most kStageBytes, in
+// slabs of whole z-planes.
+constexpr int64_t kStageBytes = int64_t(2) << 30;
+__host__ __device__ inline int stage_planes(int N) {
+  const int64_t plane = (int64_t)N * N * 10 * (int64_t)sizeof(double);
+  const int64_t n = kStageBytes / plane;
+  return n < 1 ? 1 : (n > N ? N : (int)n);
+}
 void launch_aos_to_rb(bool fp32, const ReplicaParams *P, const ReplicaState *S, int replica,
-                      int N, const double *aos, hipStream_t st);
+                      int N, int z0, int nz, const double *aos, hipStream_t st);
 void launch_rb_to_aos(bool fp32, const ReplicaParams *P, const ReplicaState *S, int replica,
-                      int N, double *aos, hipStream_t st);
+                      int N, int z0, int nz, double *aos, hipStream_t st);
 void launch_wall_dft(int N, const double *wall, const double *wallphase, double *wall_k,
                      double *wallphase_k, hipStream_t st);
 void launch_measure(bool fp32, const ReplicaParams *P, const ReplicaState *S, int replica,

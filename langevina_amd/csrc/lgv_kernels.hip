@@ -942,15 +942,14 @@ __global__ void k_set_time(ReplicaState *S, int nrep, int replica, double t) {
 // o4fields [z][y][x][10] <-> red-black planes of the live buffer.
 template <typename T>
 __global__ void k_aos_to_rb(const ReplicaParams *P, const ReplicaState *S, int replica,
-                            const double *aos) {
-  const ReplicaParams p = P[replica];
+                            int z0, int nz, const double *aos) {
+  const ReplicaParams &p = P[replica];
   const int cur = live(S[replica]);
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t V = 2 * p.half;
-  if (s >= V) return;
   const int N = p.N;
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // within the slab
+  if (s >= (int64_t)nz * N * N) return;
   const int x = (int)(s % N);
-  const int64_t yz = s / N;
+  const int64_t yz = s / N + (int64_t)z0 * N;
   const int y = (int)(yz % N), z = (int)(yz / N);
   const int par = (x + y + z) & 1;
   const int64_t i = yz * p.NH + (x >> 1);
@@ -959,15 +958,14 @@ __global__ void k_aos_to_rb(const ReplicaParams *P, const ReplicaState *S, int r
 
 template <typename T>
 __global__ void k_rb_to_aos(const ReplicaParams *P, const ReplicaState *S, int replica,
-                            double *aos) {
-  const ReplicaParams p = P[replica];
+                            int z0, int nz, double *aos) {
+  const ReplicaParams &p = P[replica];
   const int cur = live(S[replica]);
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t V = 2 * p.half;
-  if (s >= V) return;
   const int N = p.N;
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // within the slab
+  if (s >= (int64_t)nz * N * N) return;
   const int x = (int)(s % N);
-  const int64_t yz = s / N;
+  const int64_t yz = s / N + (int64_t)z0 * N;
   const int y = (int)(yz % N), z = (int)(yz / N);
   const int par = (x + y + z) & 1;
   const int64_t i = yz * p.NH + (x >> 1);
@@ -1241,17 +1239,17 @@ void launch_set_time(int nrep, int replica, ReplicaState *S, double t, hipStream
 }
 
 void launch_aos_to_rb(bool fp32, const ReplicaParams *P, const ReplicaState *S, int replica,
-                      int N, const double *aos, hipStream_t st) {
-  const int64_t V = (int64_t)N * N * N;
-  if (fp32) k_aos_to_rb<float><<<nblk(V, 256), 256, 0, st>>>(P, S, replica, aos);
-  else k_aos_to_rb<double><<<nblk(V, 256), 256, 0, st>>>(P, S, replica, aos);
+                      int N, int z0, int nz, const double *aos, hipStream_t st) {
+  const int64_t V = (int64_t)nz * N * N;
+  if (fp32) k_aos_to_rb<float><<<nblk(V, 256), 256, 0, st>>>(P, S, replica, z0, nz, aos);
+  else k_aos_to_rb<double><<<nblk(V, 256), 256, 0, st>>>(P, S, replica, z0, nz, aos);
 }
 
 void launch_rb_to_aos(bool fp32, const ReplicaParams *P, const ReplicaState *S, int replica,
-                      int N, double *aos, hipStream_t st) {
-  const int64_t V = (int64_t)N * N * N;
-  if (fp32) k_rb_to_aos<float><<<nblk(V, 256), 256, 0, st>>>(P, S, replica, aos);
-  else k_rb_to_aos<double><<<nblk(V, 256), 256, 0, st>>>(P, S, replica, aos);
+                      int N, int z0, int nz, double *aos, hipStream_t st) {
+  const int64_t V = (int64_t)nz * N * N;
+  if (fp32) k_rb_to_aos<float><<<nblk(V, 256), 256, 0, st>>>(P, S, replica, z0, nz, aos);
+  else k_rb_to_aos<double><<<nblk(V, 256), 256, 0, st>>>(P, S, replica, z0, nz, aos);
 }
 
 void launch_measure(bool fp32, const ReplicaParams *P, const ReplicaState *S, int replica,

@@ -54,6 +54,7 @@ def _create(L, params, **kw):
     cfg.replicas = C.cast(arr, C.POINTER(ModelParams))
     cfg.precision = kw.get("precision", 0)
     cfg.scounts[0], cfg.scounts[1] = kw.get("scounts", (2, 3))
+    cfg.device = kw.get("device", 0)
     h = C.c_void_p()
     rc = L.lib().lgv_create(C.byref(cfg), C.byref(h))
     return rc, L.lib().lgv_last_error().decode()
@@ -80,6 +81,14 @@ def test_create_rejects_bad_config(L):
     lib = L.lib()
     assert lib.lgv_create(None, None) == -1
     assert lib.lgv_step(None, C.c_double(0.04)) == -1
+
+
+def test_create_failure_keeps_the_first_error(L):
+    """A HIP failure inside lgv_create (here: a device index that does not
+    exist) returns LGV_ERR_HIP with the failing call in the message; the
+    partly built context is released without overwriting that message."""
+    rc, msg = _create(L, [L.model_params(16)], device=999)
+    assert rc == -2 and "hipSetDevice" in msg, msg
 
 
 def test_em_create_rejects_bad_arguments(L):
