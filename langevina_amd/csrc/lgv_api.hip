@@ -213,7 +213,12 @@ int build_charge_plan(lgv_ctx *c) {
     work.push_back((int64_t)((p.NH + TXH - 1) / TXH) * ((p.N + TQ - 1) / TQ) *
                    ((p.N + C - 1) / C) * kBlock);
   }
-  return build_plan(c->plan_charge, work, c->wide);
+  const int rc = build_plan(c->plan_charge, work, c->wide);
+  // the x sub-sweeps with whole rows per wavefront need NH = 64 or 128
+  bool xw = true;
+  for (int r = 0; r < c->nrep; r++) xw = xw && (c->hp[r].NH == 64 || c->hp[r].NH == 128);
+  c->plan_charge.xwave = xw;
+  return rc;
 }
 
 // Step coefficients with the reference's expressions:

@@ -105,6 +105,7 @@ struct Plan {
   int *d_blk_first = nullptr;
   bool wide = false;    // some plane of the launch is >= 4 GiB (64-bit offsets)
   bool stream = false;  // outputs written with non-temporal stores (see kStreamBytes)
+  bool xwave = false;   // charge launch: every replica has N = 128 or 256 (x rows per wave)
   std::vector<int> first;  // host copy of d_blk_first
 };
 

@@ -462,6 +462,72 @@ __device__ __forceinline__ void charge_line(const ReplicaParams &p, int cur, int
   }
 }
 
+// The x direction with a whole row in one wavefront (NH = 64 H): lane l
+// owns the second-parity sites A'_h of half-columns h = l + 64 j, j < H.
+// First sub-sweep: the lane updates the pair (L_h, A'_h), L_h = A'_h - e_x.
+// The updated L_{h+1} is B'_h = A'_h + e_x; it comes from the lane that owns
+// h + 1 (lane l + 1, or for lane 63 lane 0 at j + 1, the row wrapping
+// periodically).  Second sub-sweep: the lane updates (A'_h, B'_h) and
+// writes both.  Every pair is computed once and counted once, from the same
+// counter streams as charge_line.  Index logic checked on the CPU against
+// the plain two sub-sweeps (exact integer stand-in, N = 128, 256).
+template <typename T, bool W, int H>
+__device__ __forceinline__ void charge_row_wave(const ReplicaParams &p, int cur, int nxt, int e1,
+                                                int e2, uint32_t y, uint32_t z, int lane,
+                                                uint64_t k1, uint64_t k2, uint64_t &cnt,
+                                                double &dSd, double &dSu) {
+  const uint32_t N = p.N, NH = p.NH;
+  const uint32_t row = z * N + y;
+  const ChargeK ck = charge_k(p.rms, p.chi);
+  // a1: updated A'_h; r: updated L_{h+1} from lane + 1; l0: lane 0's updated L
+  double a1[H][6], r[H][6], l0[H][6];
+#pragma unroll
+  for (int j = 0; j < H; j++) {
+    const uint32_t h = lane + 64 * j;
+    const uint32_t x = site_x(h, y, z, e2);
+    const uint32_t xl = x == 0 ? N - 1 : x - 1;
+    const uint32_t iA = row * NH + h, iL = row * NH + (xl >> 1);
+    double wl[6], wa[6];
+#pragma unroll
+    for (int L = 4; L < 10; L++) {
+      wl[L - 4] = gplane<T, W>(p, cur, L, e1).ld(iL);
+      wa[L - 4] = gplane<T, W>(p, cur, L, e2).ld(iA);
+    }
+    Xoro gL = xoro_seeded(site_key(k1, row * N + xl));
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      double tl, ta;
+      charge_pair(wl[c], wa[c], ck, gL, tl, ta, cnt, dSd, dSu, true);
+      // the first sub-sweep's results as stored (rounded in FP32 storage)
+      tl = (double)(T)tl;
+      a1[j][c] = (double)(T)ta;
+      r[j][c] = __shfl(tl, (lane + 1) & 63, 64);
+      {  // lane 0's value, broadcast through scalar registers
+        const long long bits = __double_as_longlong(tl);
+        const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)bits);
+        const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(bits >> 32));
+        l0[j][c] = __longlong_as_double(((long long)hi << 32) | lo);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < H; j++) {
+    const uint32_t h = lane + 64 * j;
+    const uint32_t x = site_x(h, y, z, e2);
+    const uint32_t xb = x + 1 == N ? 0 : x + 1;
+    Xoro gA = xoro_seeded(site_key(k2, row * N + x));
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      // lane 63's B'_h is lane 0's L of the next j (wrapping to j = 0)
+      const double nb = lane == 63 ? l0[(j + 1) % H][c] : r[j][c];
+      double a2, b2;
+      charge_pair(a1[j][c], nb, ck, gA, a2, b2, cnt, dSd, dSu, true);
+      gplane<T, W>(p, nxt, 4 + c, e2).st(row * NH + h, a2);
+      gplane<T, W>(p, nxt, 4 + c, e1).st(row * NH + (xb >> 1), b2);
+    }
+  }
+}
+
 // The same two sub-sweeps with every site loaded once and every pair
 // computed once (except one first-parity pair per chunk end).  Blocks tile
 // the lattice as TXH half-columns x TQ lines x a chunk of C sites (charge_tile):
@@ -535,7 +601,7 @@ __device__ __forceinline__ void charge_column(const ReplicaParams &p, int cur, i
   }
 }
 
-template <typename T, bool W>
+template <typename T, bool W, bool XW>
 __global__ __launch_bounds__(kBlock) void k_charge_march(const ReplicaParams *__restrict__ P,
                                                          const ReplicaState *__restrict__ S,
                                                          const int *__restrict__ blk_rep,
@@ -565,7 +631,21 @@ __global__ __launch_bounds__(kBlock) void k_charge_march(const ReplicaParams *__
     const int e1 = s.order[3], e2 = s.order[4];
     const uint64_t k1 = stream_key(p.base_key, s.sweep_ctr + ctr_off);
     const uint64_t k2 = stream_key(p.base_key, s.sweep_ctr + ctr_off + 1);
-    if (dir == 0) {
+    if (XW && dir == 0) {  // every replica of the launch has NH = 64 or 128
+      // whole rows in one wavefront (TXH = 64, TQ = 4: wave w takes row q);
+      // the chunk's planes are split over the nbx x-blocks
+      const int lane = t & 63;
+      const int len = c1 - c0;
+      const int zs = c0 + bx * len / nbx, ze = c0 + (bx + 1) * len / nbx;
+      for (int z = zs; z < ze; z++) {
+        if (NH == 64)
+          charge_row_wave<T, W, 1>(p, cur, nxt, e1, e2, q, (uint32_t)z, lane, k1, k2, cnt, dSd,
+                                   dSu);
+        else
+          charge_row_wave<T, W, 2>(p, cur, nxt, e1, e2, q, (uint32_t)z, lane, k1, k2, cnt, dSd,
+                                   dSu);
+      }
+    } else if (dir == 0) {
       for (int z = c0; z < c1; z++)
         charge_line<T, W>(p, cur, nxt, 0, e1, e2, xh, q, (uint32_t)z, k1, k2, cnt, dSd, dSu);
     } else {
@@ -1201,8 +1281,20 @@ void launch_shuffle(int nrep, ReplicaState *S, const ReplicaParams *P, int ctr_o
 
 void launch_charge_march(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
                          int nrep, int ixyz, int ctr_off, MonSlot *mon, hipStream_t st) {
-  LGV_DISPATCH(k_charge_march, plan.nblocks, st, P, S, plan.d_blk_rep, plan.d_blk_first, ixyz,
-               ctr_off, mon);
+  // the row-per-wavefront x path costs registers (3 waves per SIMD instead of
+  // 4), so it is compiled in only for launches whose replicas all use it
+  const int *br = plan.d_blk_rep, *bf = plan.d_blk_first;
+  const int g = plan.nblocks;
+#define LGV_CHARGE(TT, WW, XX)                                                         \
+  k_charge_march<TT, WW, XX><<<g, kBlock, 0, st>>>(P, S, br, bf, ixyz, ctr_off, mon)
+  if (plan.xwave) {
+    if (fp32) { if (plan.wide) LGV_CHARGE(float, true, true); else LGV_CHARGE(float, false, true); }
+    else { if (plan.wide) LGV_CHARGE(double, true, true); else LGV_CHARGE(double, false, true); }
+  } else {
+    if (fp32) { if (plan.wide) LGV_CHARGE(float, true, false); else LGV_CHARGE(float, false, false); }
+    else { if (plan.wide) LGV_CHARGE(double, true, false); else LGV_CHARGE(double, false, false); }
+  }
+#undef LGV_CHARGE
   k_flip_chg<<<nblk(nrep, 64), 64, 0, st>>>(S, nrep);
 }
 

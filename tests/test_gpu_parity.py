@@ -280,6 +280,29 @@ def test_step_variants(L, orc, variant):
     ctx.close()
 
 
+@pytest.mark.parametrize("N, precision", [(128, 0), (128, 1), (256, 0)])
+def test_charge_rows_in_one_wavefront(L, orc, N, precision):
+    """At N = 128 and 256 a wavefront owns whole x-rows of the charge sweep
+    (first-parity results passed between lanes); the charge-only step gives
+    the oracle's bits."""
+    log("charge rows", N, precision)
+    ctx = L.Context(L.model_params(N, seed=6, **PAR), precision=precision, onlydiffuse=True)
+    lat = orc.Lattice(N, seed=6, rng_mode=orc.RNG_PER_SITE, precision=precision,
+                      onlydiffuse=True, mass0=PAR["mass0"], lam=PAR["lam"], H=PAR["H"],
+                      chi=PAR["chi"])
+    u = np.random.default_rng(6).normal(size=(N, N, N, 10)) * 0.8
+    if precision == 1:
+        u = u.astype(np.float32).astype(np.float64)
+    lat.u[...] = u
+    ctx.set_field(u)
+    del u
+    for _ in range(2):
+        ctx.step(0.04)
+        lat.step(0.04)
+    np.testing.assert_array_equal(ctx.get_field(), lat.u)
+    ctx.close()
+
+
 def test_lattice_spacing_and_moving_mass(L, orc):
     """LX = 2 NX (h = 2, the kick's 1/h^2 factor, ModelA.h:229) and
     m^2(t) = mass0 + dmassdt t."""
