@@ -739,32 +739,38 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
   // plane z+1 before the kick of plane z, so that their latency overlaps the
   // kick (free at FP64's 2 waves per SIMD; FP32 would drop from 3 waves to 2)
   constexpr bool kEarlyRing = sizeof(T) == 8;
+  // The thread's two own sites of a plane are indexed by parity (pp), not by
+  // x offset: a parity's plane base is the same for every lane, so the loads
+  // and stores use one scalar base and 32-bit offsets.  The site of parity pp
+  // sits at x = 2 xh + ((pp + y + z) & 1); its z +- 1 neighbours have parity
+  // pp ^ 1 in the same half-column.
   double cen[2][4], nex[2][4];  // rotated phi at the own sites of planes z, z+1
   double chg[2][6], chn[2][6];  // raw charges of planes z, z+1 (kCarry only)
   auto rotate_own = [&](int z, double out[2][4], double ch[2][6]) __attribute__((always_inline)) {
     if (!own) return;
     const uint32_t i = ((uint32_t)z * N + y) * NH + xh;
 #pragma unroll
-    for (int s = 0; s < 2; s++) {
-      const int par = (s + y + z) & 1;
+    for (int pp = 0; pp < 2; pp++) {
       double f[10];
 #pragma unroll
-      for (int c = 0; c < 10; c++) f[c] = gplane<T, W>(p, cur, c, par).ld(i);
+      for (int c = 0; c < 10; c++) f[c] = gplane<T, W>(p, cur, c, pp).ld(i);
       rotate_site_fast(f, f + 4, f + 7, h);
 #pragma unroll
-      for (int c = 0; c < 4; c++) out[s][c] = f[c];
+      for (int c = 0; c < 4; c++) out[pp][c] = f[c];
       if constexpr (kCarry) {
 #pragma unroll
-        for (int c = 0; c < 6; c++) ch[s][c] = f[4 + c];
+        for (int c = 0; c < 6; c++) ch[pp][c] = f[4 + c];
       }
     }
   };
   auto fill_lds = [&](int z, const double o[2][4]) __attribute__((always_inline)) {
     if (own) {
 #pragma unroll
-      for (int s = 0; s < 2; s++)
+      for (int pp = 0; pp < 2; pp++) {
+        const int sx = (pp + y + z) & 1;
 #pragma unroll
-        for (int c = 0; c < 4; c++) lds[c][(ty + 1) * LW + (2 * tx + s + 1)] = o[s][c];
+        for (int c = 0; c < 4; c++) lds[c][(ty + 1) * LW + (2 * tx + sx + 1)] = o[pp][c];
+      }
     }
     auto ring_fill = [&](int gx, int gy, int pos) __attribute__((always_inline)) {
       double f[10];
@@ -808,30 +814,30 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
     if (own) {
       const uint32_t i = ((uint32_t)z * N + y) * NH + xh;
 #pragma unroll
-      for (int s = 0; s < 2; s++) {
+      for (int pp = 0; pp < 2; pp++) {
         // held to 1e-12 like the rotations: contraction into FMA allowed
 #pragma clang fp contract(fast)
-        const int lx = 2 * tx + s + 1, ly = ty + 1;
+        const int lx = 2 * tx + ((pp + y + z) & 1) + 1, ly = ty + 1;
         double cp[4], xp[4], xm[4], yp[4], ym[4];
 #pragma unroll
         for (int c = 0; c < 4; c++) {
-          cp[c] = cen[s][c];
+          cp[c] = cen[pp][c];
           xp[c] = lds[c][ly * LW + lx + 1];
           xm[c] = lds[c][ly * LW + lx - 1];
           yp[c] = lds[c][(ly + 1) * LW + lx];
           ym[c] = lds[c][(ly - 1) * LW + lx];
         }
-        const int par = (s + y + z) & 1;
-        const double *zp = nex[s];
+        const int par = pp;
+        const double *zp = nex[pp ^ 1];
         double zm[4], A[3], V[3];
 #pragma unroll
-        for (int c = 0; c < 4; c++) zm[c] = lprv[s][c][t];
+        for (int c = 0; c < 4; c++) zm[c] = lprv[pp ^ 1][c][t];
         // the charges of plane z, unchanged by the first rotation
 #pragma unroll
         for (int l = 0; l < 3; l++) {
           if constexpr (kCarry) {
-            A[l] = chg[s][l];
-            V[l] = chg[s][3 + l];
+            A[l] = chg[pp][l];
+            V[l] = chg[pp][3 + l];
           } else {
             A[l] = gplane<T, W>(p, cur, 4 + l, par).ld(i);
             V[l] = gplane<T, W>(p, cur, 7 + l, par).ld(i);
