@@ -93,6 +93,7 @@ struct lgv_ctx {
   Plan plan_half, plan_full, plan_ideal, plan_charge, plan_hb;
   bool wide = false;    // some plane is >= 4 GiB: kernels use 64-bit offsets
   bool nt_stores = false;  // one buffer's fields exceed kStreamBytes
+  bool phi_march = false;  // the phi half-sweep runs the z-march kernel
   MonSlot *mon_phi = nullptr, *mon_charge = nullptr, *mon_ideal = nullptr;
   EnergySlot *eslots = nullptr;
   double coef_hb = NAN, coef_charge = NAN, coef_ideal = NAN;
@@ -324,13 +325,13 @@ void enqueue_ideal(lgv_ctx *c, int off) {
 // 85 against 75, 128^3 35 against 29, 64^3 30 against 9 (below 1 GiB the
 // fields stay largely cache-resident and the per-site kernel's 8 waves per
 // SIMD win); FP32 256^3 114 against 111.
-bool use_march(const lgv_ctx *c) { return !c->fp32 && c->nt_stores; }
+bool use_march(const lgv_ctx *c) { return c->phi_march; }
 const Plan &hb_plan(const lgv_ctx *c) { return use_march(c) ? c->plan_hb : c->plan_half; }
 void enqueue_hb(lgv_ctx *c, int off) {
   for (int ieo = 0; ieo < 2; ieo++) {
     Timed tm(c, LGV_CLASS_PHI);
     if (use_march(c))
-      launch_hb_march(false, c->plan_hb, c->dP, c->dS, ieo, off + ieo, c->mon_phi, c->stream);
+      launch_hb_march(c->fp32, c->plan_hb, c->dP, c->dS, ieo, off + ieo, c->mon_phi, c->stream);
     else
       launch_hb_phi(c->fp32, c->plan_half, c->dP, c->dS, ieo, off + ieo, c->mon_phi, c->stream);
   }
@@ -469,6 +470,14 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
   c->nt_stores = buffer_bytes > kStreamBytes;
   // test hook: LGV_NT_STORES=0/1 forces the store mode on any size
   if (const char *nt = getenv("LGV_NT_STORES")) c->nt_stores = atoi(nt) != 0;
+  c->phi_march = !c->fp32 && c->nt_stores;
+  // test hook: LGV_PHI_KERNEL=site/march forces the phi half-sweep kernel
+  if (const char *pk = getenv("LGV_PHI_KERNEL")) {
+    const std::string k(pk);
+    if (k != "site" && k != "march")
+      return fail(LGV_ERR_ARG, "LGV_PHI_KERNEL=%s: expected site or march", pk);
+    c->phi_march = k == "march";
+  }
   HIPCHECK(hipMalloc(&c->dP, c->nrep * sizeof(ReplicaParams)));
   HIPCHECK(hipMemcpy(c->dP, c->hp.data(), c->nrep * sizeof(ReplicaParams),
                      hipMemcpyHostToDevice));

@@ -371,6 +371,42 @@ def test_nontemporal_store_kernels_match(L, monkeypatch):
         np.testing.assert_array_equal(out["0", prec], out["1", prec])
 
 
+@pytest.mark.parametrize("N", [8, 12, 20, 132])
+def test_phi_half_sweep_kernels_match(L, monkeypatch, N):
+    """The two phi half-sweep kernels (one thread per site, the z-march with
+    the other parity's plane in LDS) give the same bits and the same
+    acceptance counts, in both storage precisions.  N = 132 puts several
+    64 x 4 x 16 march tiles on the lattice, the last of them short in x and
+    z; for N <= 20 one tile spans the whole cross-section, so its ring wraps
+    onto its own half-columns."""
+    log(f"phi sweep kernels N={N}")
+    u = np.random.default_rng(N).normal(size=(N, N, N, 10)) * 0.6
+    out, mons = {}, {}
+    for kern in ("site", "march"):
+        monkeypatch.setenv("LGV_PHI_KERNEL", kern)
+        for prec in (0, 1):
+            ctx = L.Context(L.model_params(N, seed=6, **PAR), precision=prec)
+            ctx.set_field(u.astype(np.float32).astype(np.float64) if prec else u)
+            ctx.run(0.04, 2)
+            out[kern, prec] = ctx.get_field()
+            mons[kern, prec] = ctx.monitors()["phi"]
+            ctx.close()
+    for prec in (0, 1):
+        np.testing.assert_array_equal(out["march", prec], out["site", prec])
+        a, b = mons["march", prec], mons["site", prec]
+        for k in a:
+            if isinstance(a[k], int):
+                assert a[k] == b[k], (prec, k)
+            else:
+                assert a[k] == pytest.approx(b[k], rel=1e-9, abs=1e-12), (prec, k)
+
+
+def test_phi_kernel_choice_is_checked(L, monkeypatch):
+    monkeypatch.setenv("LGV_PHI_KERNEL", "fused")
+    with pytest.raises(L.LgvError, match="expected site or march"):
+        L.Context(L.model_params(8, seed=1, **PAR))
+
+
 def test_field_transfer_in_slabs(L, orc, monkeypatch):
     """set_field / get_field move the AoS field through a staging buffer in
     slabs of z-planes (one slab up to 2 GiB); forced to 3-plane slabs (the last
