@@ -699,9 +699,23 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
   ideal_tile(N, p.tile_cap, p.tile_tz, TXH, TY, TZ);
   const int ntx = (NH + TXH - 1) / TXH, nty = (N + TY - 1) / TY;
   int b = bid - blk_first[r];
-  const int bx = b % ntx;
-  b /= ntx;
-  const int by = b % nty, bz = b / nty;
+  int bx, by, bz;
+  const int nz = (N + TZ - 1) / TZ;
+  if ((int)gridDim.x == ntx * nty * nz && (nty & 7) == 0) {
+    // one replica: XCD b % 8 (round-robin dispatch) sweeps the y band of
+    // tiles nty/8 * (b % 8) .. + nty/8, all x tiles, z chunk after z chunk,
+    // so the tiles that share a ring run together in one L2
+    const int nyb = nty >> 3, i8 = b >> 3;
+    bx = i8 % ntx;
+    const int t8 = i8 / ntx;
+    by = (b & 7) * nyb + t8 % nyb;
+    bz = t8 / nyb;
+  } else {
+    bx = b % ntx;
+    b /= ntx;
+    by = b % nty;
+    bz = b / nty;
+  }
   const int xh0 = bx * TXH, y0 = by * TY, z0 = bz * TZ;
   const int tw = min(TXH, NH - xh0), th = min(TY, N - y0), tz = min(TZ, N - z0);
   const int LW = 2 * tw + 2;  // LDS row width
