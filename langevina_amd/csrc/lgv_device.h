@@ -18,6 +18,27 @@ struct Xoro {
   uint64_t s0, s1;
 };
 
+// Tile (bx, by, bz) of march block b in a grid of exactly ntx x nty x nz
+// blocks.  When nty is a multiple of 8, XCD b % 8 (the dispatcher deals
+// workgroups round-robin over the 8 XCDs) sweeps the y band of tiles
+// nty/8 * (b % 8) .. + nty/8, all x tiles, z chunk after z chunk: the tiles
+// that share ring lines run together in one L2, and the eight XCDs stream
+// from eight row bands of every plane.  Otherwise x, then y, then z.
+__device__ __forceinline__ void yband_tile(int b, int ntx, int nty, int &bx, int &by, int &bz) {
+  if ((nty & 7) == 0) {
+    const int nyb = nty >> 3, i8 = b >> 3;
+    bx = i8 % ntx;
+    const int t8 = i8 / ntx;
+    by = (b & 7) * nyb + t8 % nyb;
+    bz = t8 / nyb;
+  } else {
+    bx = b % ntx;
+    b /= ntx;
+    by = b % nty;
+    bz = b / nty;
+  }
+}
+
 LGV_HD uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
 
 // xoroshiro128plus::seed(uint64_t), ModelG/xoroshiro128plus.h:86-91

@@ -107,22 +107,8 @@ struct EmMarch {
     int TX, TY, TZ;
     em_tile(N, TX, TY, TZ);
     const int ntx = (N + TX - 1) / TX, nty = (N + TY - 1) / TY;
-    int b = blockIdx.x, bx, by, bz;
-    if ((nty & 7) == 0) {
-      // XCD b % 8 (round-robin dispatch) sweeps the y band of tiles
-      // nty/8 * (b % 8) .. + nty/8, all x tiles, z chunk after z chunk, so
-      // the tiles that share a ring run together in one L2
-      const int nyb = nty >> 3, i8 = b >> 3;
-      bx = i8 % ntx;
-      const int t8 = i8 / ntx;
-      by = (b & 7) * nyb + t8 % nyb;
-      bz = t8 / nyb;
-    } else {
-      bx = b % ntx;
-      b /= ntx;
-      by = b % nty;
-      bz = b / nty;
-    }
+    int bx, by, bz;
+    yband_tile(blockIdx.x, ntx, nty, bx, by, bz);
     x0 = bx * TX;
     y0 = by * TY;
     z0 = bz * TZ;

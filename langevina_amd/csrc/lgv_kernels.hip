@@ -75,18 +75,10 @@ __device__ __forceinline__ GPlane<T, W> gplane(const ReplicaParams &p, int cur, 
 
 // Block index of this workgroup within the launch plan.  (A renumbering that
 // gives each XCD a contiguous eighth of the plan was measured on MI355X and
-// made the ideal and charge sweeps slower; they run in plain dispatch order.)
+// made the ideal and charge sweeps slower; the ideal kernel and the phi march
+// map one-replica launches with yband_tile instead, the charge sweep keeps
+// dispatch order.)
 __device__ __forceinline__ int xcd_block() { return blockIdx.x; }
-
-// The renumbering itself, used by the phi march: the dispatcher deals
-// workgroups round-robin over the 8 XCDs, so workgroup b runs on XCD b % 8;
-// plan block (b % 8) * (G/8) + b / 8 then puts x-adjacent tiles, which share
-// the halo columns' cache lines, into one XCD's L2.  A bijection on the first
-// 8 * (G/8) blocks; the tail keeps its index.
-__device__ __forceinline__ int xcd_contiguous_block() {
-  const int b = blockIdx.x, g8 = (int)gridDim.x >> 3;
-  return b < (g8 << 3) ? (b & 7) * g8 + (b >> 3) : b;
-}
 
 __device__ __forceinline__ int wrapp(int a, int N) { return a == N ? 0 : a; }
 __device__ __forceinline__ int wrapm(int a, int N) { return a < 0 ? N - 1 : a; }
@@ -247,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_hb_march(const ReplicaParams *__rest
                                                      int ctr_off, MonSlot *__restrict__ mon) {
   constexpr int kGrid = kHbMarchGrid;
   __shared__ T Q[2][4][kGrid];
-  const int bid = xcd_contiguous_block();
+  const int bid = xcd_block();
   const int r = blk_rep[bid];
   const ReplicaParams &p = P[r];
   const ReplicaState &s = S[r];
@@ -258,9 +250,15 @@ __global__ __launch_bounds__(kBlock) void k_hb_march(const ReplicaParams *__rest
   hb_tile(N, TXH, TY, TZ);
   const int ntx = (NH + TXH - 1) / TXH, nty = (N + TY - 1) / TY;
   int b = bid - blk_first[r];
-  const int bx = b % ntx;
-  b /= ntx;
-  const int by = b % nty, bz = b / nty;
+  int bx, by, bz;
+  if ((int)gridDim.x == ntx * nty * ((N + TZ - 1) / TZ)) {
+    yband_tile(b, ntx, nty, bx, by, bz);  // one replica
+  } else {
+    bx = b % ntx;
+    b /= ntx;
+    by = b % nty;
+    bz = b / nty;
+  }
   const int xh0 = bx * TXH, y0 = by * TY, z0 = bz * TZ;
   const int tw = min(TXH, NH - xh0), th = min(TY, N - y0), tz = min(TZ, N - z0);
   const int GW = tw + 2;  // grid row width: half-columns xh0-1 .. xh0+tw
@@ -700,16 +698,8 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
   const int ntx = (NH + TXH - 1) / TXH, nty = (N + TY - 1) / TY;
   int b = bid - blk_first[r];
   int bx, by, bz;
-  const int nz = (N + TZ - 1) / TZ;
-  if ((int)gridDim.x == ntx * nty * nz && (nty & 7) == 0) {
-    // one replica: XCD b % 8 (round-robin dispatch) sweeps the y band of
-    // tiles nty/8 * (b % 8) .. + nty/8, all x tiles, z chunk after z chunk,
-    // so the tiles that share a ring run together in one L2
-    const int nyb = nty >> 3, i8 = b >> 3;
-    bx = i8 % ntx;
-    const int t8 = i8 / ntx;
-    by = (b & 7) * nyb + t8 % nyb;
-    bz = t8 / nyb;
+  if ((int)gridDim.x == ntx * nty * ((N + TZ - 1) / TZ)) {
+    yband_tile(b, ntx, nty, bx, by, bz);  // one replica
   } else {
     bx = b % ntx;
     b /= ntx;

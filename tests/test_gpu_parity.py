@@ -371,14 +371,15 @@ def test_nontemporal_store_kernels_match(L, monkeypatch):
         np.testing.assert_array_equal(out["0", prec], out["1", prec])
 
 
-@pytest.mark.parametrize("N", [8, 12, 20, 132])
+@pytest.mark.parametrize("N", [8, 12, 20, 64, 132])
 def test_phi_half_sweep_kernels_match(L, monkeypatch, N):
     """The two phi half-sweep kernels (one thread per site, the z-march with
     the other parity's plane in LDS) give the same bits and the same
     acceptance counts, in both storage precisions.  N = 132 puts several
     64 x 4 x 16 march tiles on the lattice, the last of them short in x and
-    z; for N <= 20 one tile spans the whole cross-section, so its ring wraps
-    onto its own half-columns."""
+    z; N = 64 has 8 rows of 32 x 8 tiles, so the march numbers its blocks by
+    y bands (one band per XCD); for N <= 20 one tile spans the whole
+    cross-section, so its ring wraps onto its own half-columns."""
     log(f"phi sweep kernels N={N}")
     u = np.random.default_rng(N).normal(size=(N, N, N, 10)) * 0.6
     out, mons = {}, {}
