@@ -41,10 +41,10 @@ SCHEDULE_BYTES_PER_SITE_UPDATE_FP32 = SCHEDULE_BYTES_PER_SITE_UPDATE
 # copy-kernel calibration (bench.py --calibrate-copy); the summary of those
 # runs is profiles/modelg_256_hbm_traffic.md.
 MEASURED_TRAFFIC = {
-    ("k_ideal_fused", "fp64", 256): (145.04, 80.00),
-    ("k_hb_march", "fp64", 256): (34.76, 16.01),
+    ("k_ideal_fused", "fp64", 256): (95.15, 80.00),
+    ("k_hb_march", "fp64", 256): (36.34, 16.01),
     ("k_charge_march", "fp64", 256): (51.91, 48.23),
-    ("k_ideal_fused", "fp32", 256): (117.43, 40.00),
+    ("k_ideal_fused", "fp32", 256): (76.76, 40.00),
     ("k_hb_phi", "fp32", 256): (24.57, 8.12),
     ("k_charge_march", "fp32", 256): (25.70, 24.14),
 }

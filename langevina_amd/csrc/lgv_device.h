@@ -39,6 +39,15 @@ __device__ __forceinline__ void yband_tile(int b, int ntx, int nty, int &bx, int
   }
 }
 
+// Plan block of workgroup b for a one-thread-per-site launch that covers
+// whole planes in bpp blocks each (bpp a multiple of 8): XCD b % 8 takes the
+// (b % 8)-th eighth of every plane (a band of rows), plane after plane, so a
+// site's y and z neighbours are read through the same L2.
+__device__ __forceinline__ int band_block(int b, int bpp) {
+  const int S = bpp >> 3, i = b >> 3, c = i / S;
+  return (c * 8 + (b & 7)) * S + (i - c * S);
+}
+
 LGV_HD uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
 
 // xoroshiro128plus::seed(uint64_t), ModelG/xoroshiro128plus.h:86-91
