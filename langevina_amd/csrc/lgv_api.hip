@@ -322,7 +322,8 @@ void enqueue_ideal(lgv_ctx *c, int off) {
 // The phi half-sweep kernel: the z-march for FP64 fields in the large-field
 // (non-temporal) mode, the one-thread-per-site kernel otherwise.  Measured
 // per half-sweep, FP64: 256^3 march 157 us against 204 us per site; 192^3
-// 85 against 75, 128^3 35 against 29, 64^3 30 against 9 (below 1 GiB the
+// 85 against 75 (70 in plane bands), 128^3 35 against 29 (26), 64^3 30
+// against 9 (below 1 GiB the
 // fields stay largely cache-resident and the per-site kernel's 8 waves per
 // SIMD win); FP32 256^3 114 against 111.
 bool use_march(const lgv_ctx *c) { return c->phi_march; }
