@@ -187,10 +187,22 @@ def test_reference_and_per_site_modes_share_structure(orc):
     # the same driver sequence consumes counters identically in both modes
     a = orc.Lattice(4, rng_mode=orc.RNG_REFERENCE, seed=3)
     b = orc.Lattice(4, rng_mode=orc.RNG_PER_SITE, seed=3)
+    init = []
     for lat in (a, b):
         lat.init_gaussian_charges()
+        init.append(lat.u.copy())
         lat.step(0.04)
-    np.testing.assert_array_equal(a.u[..., 4:].sum(), a.u[..., 4:].sum())
+    # both modes draw the initial charges from the global engine
+    # (ModelA.h:463-525): identical bits, zero mode removed
+    np.testing.assert_array_equal(init[0], init[1])
+    np.testing.assert_allclose(init[0][..., 4:].sum(axis=(0, 1, 2)), 0, atol=1e-12)
+    # a step moves charge between sites only: sum V is conserved in both
+    # modes (the kick is a lattice divergence, the exchange moves +-q), while
+    # the trajectories themselves differ
+    for lat in (a, b):
+        np.testing.assert_allclose(lat.u[..., 7:].sum(axis=(0, 1, 2)),
+                                   init[0][..., 7:].sum(axis=(0, 1, 2)), atol=1e-12)
+    assert not np.array_equal(a.u, b.u)
     assert b.c.sweep_ctr == 22  # 3 x (1 ideal + 2 x 2 parities) + 1 shuffle + 6 charge
     assert a.c.sweep_ctr == 0
 
