@@ -138,20 +138,8 @@ def batched_replicas(L):
     """96 independent replicas: 48 x 32^3, 32 x 48^3, 16 x 64^3; seed 1000+r;
     H_r uniform in [0.003, 0.03] from a xoroshiro128+ stream seeded with
     12345 by the reference's seed formula; m^2 = -4.813."""
-    from langevina_amd.lgv import model_params
-    sizes = [32] * 48 + [48] * 32 + [64] * 16
-    s0 = (0x01d353e5f3993bb0 + 12345) % 2**64
-    s1 = (0x7b9c0df6cb193b20 * 12346) % 2**64
-    reps = []
-    for r, N in enumerate(sizes):
-        res = (s0 + s1) % 2**64  # xoroshiro128+ output
-        t = s1 ^ s0
-        s0 = (((s0 << 55) | (s0 >> 9)) % 2**64) ^ t ^ ((t << 14) % 2**64)
-        s1 = ((t << 36) | (t >> 28)) % 2**64
-        u = float(res) / 2**64
-        H = 0.003 + (0.03 - 0.003) * u
-        reps.append(model_params(N, seed=1000 + r, **dict(PARAMS, H=H)))
-    return reps
+    from langevina_amd.lgv import ensemble_params
+    return ensemble_params([32] * 48 + [48] * 32 + [64] * 16, **PARAMS)
 
 
 # Model A Euler-Maruyama (lgv_em_*): algorithmic bytes per site-step (FP64,

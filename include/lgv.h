@@ -205,6 +205,23 @@ int lgv_em_forward(lgv_em *h, const double *phi0, int window, double *phi_final,
                    double *phibar_host, double *loss);
 /* d loss / d phi0 (device, same layout) and d loss / d (m2, H, lambda). */
 int lgv_em_backward(lgv_em *h, double *grad_phi0, double grad_params[3]);
+/* The observables of one state, 6 doubles: the volume means phibar_0..3,
+ * the volume mean of |phi|^2 and V |phibar|^2 (the per-measurement
+ * estimator whose series gives the susceptibility V (<|phibar|^2> -
+ * <|phibar|>^2)); the volume averages of Measurer, measurer.h:169-179. */
+#define LGV_EM_NOBS 6
+/* The measurement series of the run loop (measure, then step:
+ * SuperPions.cxx:124-160): the handle's nsteps steps from phi0, with the
+ * observables of the states 0, every, 2 every, ... below nsteps written to
+ * rows_host[ceil(nsteps / every)][LGV_EM_NOBS].  phi_final (device, may be
+ * NULL) receives the last state.  Leaves the forward/backward state alone. */
+int lgv_em_series(lgv_em *h, const double *phi0, int every, double *phi_final,
+                  double *rows_host);
+/* One step, number n (it selects the noise), from in to out (caller-owned
+ * device arrays, in != out), stream-ordered and without a host
+ * synchronisation.  obs (device memory, may be NULL) receives the
+ * LGV_EM_NOBS observables of out. */
+int lgv_em_step(lgv_em *h, const double *in, double *out, int n, double *obs);
 int lgv_em_destroy(lgv_em *h);
 
 #ifdef __cplusplus

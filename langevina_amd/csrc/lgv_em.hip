@@ -69,22 +69,103 @@ __device__ __forceinline__ void em_noise(uint64_t skey, int64_t s, int kind, dou
   }
 }
 
-__device__ void block_ksum_store(KSum k, double *out2) {
-  for (int o = 32; o > 0; o >>= 1) {
-    KSum t;
-    t.s = __shfl_xor(k.s, o, 64);
-    t.c = __shfl_xor(k.c, o, 64);
-    ksum_merge(k, t);
-  }
-  __shared__ KSum sh[kEmBlock / 64];
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) sh[w] = k;
+// Volume sums of one launch: NS = 0 (none), 1 (phi_0: phibar) or 5 (phi_0..3
+// and |phi|^2: the observables of a measurement).  Every block stores its
+// compensated partials in its own slot.  The slots of step n are merged,
+// in block order, by block 0 of the launch of step n+1 (the kernel
+// boundary orders the two, so no fence or atomic is needed), or by
+// k_em_merge after the last step of a run.
+constexpr int kObsSums = 5;
+constexpr int kPartStride = 2 * kObsSums;  // doubles per block slot
+constexpr int kNobs = 6;  // phibar_0..3, <|phi|^2>, V |phibar|^2
+static_assert(kNobs == LGV_EM_NOBS, "observable count of the C ABI");
+
+// Slots an earlier launch left, and where their merged values go.
+struct EmMerge {
+  const double *slots;  // nblocks * kPartStride doubles, or nullptr: nothing
+  int ns;               // 1 or kObsSums sums per slot
+  int nblocks;
+  double *phibar;       // volume mean of phi_0, or nullptr
+  double *obs;          // kNobs observables (ns == kObsSums), or nullptr
+};
+
+template <int NS>
+__device__ __forceinline__ void wave_merge(KSum *k) {
+#pragma unroll
+  for (int i = 0; i < NS; i++)
+    for (int o = 32; o > 0; o >>= 1) {
+      KSum t;
+      t.s = __shfl_xor(k[i].s, o, 64);
+      t.c = __shfl_xor(k[i].c, o, 64);
+      ksum_merge(k[i], t);
+    }
+}
+
+// This block's NS sums into its slot.
+template <int NS>
+__device__ void em_store_slot(KSum *acc, double *slots) {
+  __shared__ KSum sh[NS][kEmBlock / 64];
+  wave_merge<NS>(acc);
+  if ((threadIdx.x & 63) == 0)
+    for (int i = 0; i < NS; i++) sh[i][threadIdx.x >> 6] = acc[i];
   __syncthreads();
   if (threadIdx.x == 0) {
-    KSum a = sh[0];
-    for (int i = 1; i < kEmBlock / 64; i++) ksum_merge(a, sh[i]);
-    out2[0] = a.s;
-    out2[1] = a.c;
+    double *slot = slots + (size_t)blockIdx.x * kPartStride;
+    for (int i = 0; i < NS; i++) {
+      KSum a = sh[i][0];
+      for (int q = 1; q < kEmBlock / 64; q++) ksum_merge(a, sh[i][q]);
+      slot[2 * i] = a.s;
+      slot[2 * i + 1] = a.c;
+    }
+  }
+}
+
+// Merges the slots of m in block order with the whole (kEmBlock-thread)
+// block and writes phibar / obs; every thread of the block calls it.
+template <int NS>
+__device__ void em_merge_ns(const EmMerge &m, int64_t V) {
+  __shared__ KSum sh[NS][kEmBlock / 64];
+  KSum b[NS];
+  for (int i = 0; i < NS; i++) b[i] = KSum{0., 0.};
+  for (int q = threadIdx.x; q < m.nblocks; q += kEmBlock) {
+    const double *slot = m.slots + (size_t)q * kPartStride;
+    for (int i = 0; i < NS; i++) ksum_merge(b[i], KSum{slot[2 * i], slot[2 * i + 1]});
+  }
+  wave_merge<NS>(b);
+  if ((threadIdx.x & 63) == 0)
+    for (int i = 0; i < NS; i++) sh[i][threadIdx.x >> 6] = b[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v[NS];
+    for (int i = 0; i < NS; i++) {
+      KSum a = sh[i][0];
+      for (int q = 1; q < kEmBlock / 64; q++) ksum_merge(a, sh[i][q]);
+      v[i] = ksum_value(a) / (double)V;
+    }
+    if (m.phibar) *m.phibar = v[0];
+    if constexpr (NS == kObsSums) {
+      if (m.obs) {
+        for (int i = 0; i < NS; i++) m.obs[i] = v[i];
+        m.obs[5] = (double)V * (((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]) + v[3] * v[3]);
+      }
+    }
+  }
+  __syncthreads();
+}
+__device__ __forceinline__ void em_merge(const EmMerge &m, int64_t V) {
+  if (m.ns == kObsSums) em_merge_ns<kObsSums>(m, V);
+  else em_merge_ns<1>(m, V);
+}
+
+// Adds one site's phi to the NS sums.
+template <int NS>
+__device__ __forceinline__ void em_accumulate(KSum *acc, const double o[4]) {
+  if constexpr (NS >= 1) ksum_add(acc[0], o[0]);
+  if constexpr (NS == kObsSums) {
+    ksum_add(acc[1], o[1]);
+    ksum_add(acc[2], o[2]);
+    ksum_add(acc[3], o[3]);
+    ksum_add(acc[4], ((o[0] * o[0] + o[1] * o[1]) + o[2] * o[2]) + o[3] * o[3]);
   }
 }
 
@@ -140,17 +221,20 @@ struct EmMarch {
   }
 };
 
-// One forward step in -> out, marching z with the plane in LDS; per-block
-// compensated partial of sum out[0].
+// One forward step in -> out, marching z with the plane in LDS; the NS
+// volume sums of `out` go to `red`.
+template <int NS>
 __global__ __launch_bounds__(kEmBlock) void k_em_step(const double *__restrict__ in,
                                                       double *__restrict__ out, int N,
                                                       double m2, double lam, double H,
                                                       double gdt, double amp, uint64_t skey,
-                                                      int kind, double *__restrict__ part) {
+                                                      int kind, double *slots, EmMerge prev) {
   __shared__ double Q[2][4][kEmGrid];
   const int64_t V = (int64_t)N * N * N;
+  if (prev.slots && blockIdx.x == 0) em_merge(prev, V);
   const EmMarch m(N);
-  KSum acc{0., 0.};
+  KSum acc[NS > 0 ? NS : 1];
+  for (int i = 0; i < NS; i++) acc[i] = KSum{0., 0.};
   double cm[4], cc[4], cp[4];  // own column at z-1, z, z+1
   if (m.own) {
     const int zm = m.z0 == 0 ? N - 1 : m.z0 - 1;
@@ -179,7 +263,7 @@ __global__ __launch_bounds__(kEmBlock) void k_em_step(const double *__restrict__
     if (m.own) {
       const uint32_t s = m.site(m.x, m.y, z);
       const double s2 = cc[0] * cc[0] + cc[1] * cc[1] + cc[2] * cc[2] + cc[3] * cc[3];
-      double xi[4];
+      double xi[4], o[4];
       em_noise(skey, s, kind, xi);
 #pragma unroll
       for (int a = 0; a < 4; a++) {
@@ -188,10 +272,10 @@ __global__ __launch_bounds__(kEmBlock) void k_em_step(const double *__restrict__
         const double lap = nb - 6.0 * cc[a];
         double drift = (-lap + m2 * cc[a]) + (lam * s2) * cc[a];
         if (a == 0) drift = drift - H;
-        const double o = (cc[a] - gdt * drift) + amp * xi[a];
-        out[a * V + s] = o;
-        if (a == 0) ksum_add(acc, o);
+        o[a] = (cc[a] - gdt * drift) + amp * xi[a];
+        out[a * V + s] = o[a];
       }
+      em_accumulate<NS>(acc, o);
 #pragma unroll
       for (int a = 0; a < 4; a++) {
         cm[a] = cc[a];
@@ -199,31 +283,31 @@ __global__ __launch_bounds__(kEmBlock) void k_em_step(const double *__restrict__
       }
     }
   }
-  block_ksum_store(acc, part + 2 * blockIdx.x);
+  if constexpr (NS > 0) em_store_slot<NS>(acc, slots);
 }
 
-// phibar[n] = (compensated sum of the nblocks partials) / V, one block.
-constexpr int kPhibarThreads = 1024;
-__global__ __launch_bounds__(kPhibarThreads) void k_em_phibar(const double *part, int nblocks,
-                                                              int64_t V, double *phibar, int n) {
-  KSum acc{0., 0.};
-  for (int b = threadIdx.x; b < nblocks; b += kPhibarThreads)
-    ksum_merge(acc, KSum{part[2 * b], part[2 * b + 1]});
-  for (int o = 32; o > 0; o >>= 1) {
-    KSum t;
-    t.s = __shfl_xor(acc.s, o, 64);
-    t.c = __shfl_xor(acc.c, o, 64);
-    ksum_merge(acc, t);
+// The observable sums of a state without stepping it (the measurement
+// before the first step); the same blocks and slots as k_em_step.
+__global__ __launch_bounds__(kEmBlock) void k_em_observe(const double *__restrict__ in, int N,
+                                                         double *slots) {
+  const int64_t V = (int64_t)N * N * N;
+  const EmMarch m(N);
+  KSum acc[kObsSums];
+  for (int i = 0; i < kObsSums; i++) acc[i] = KSum{0., 0.};
+  if (m.own) {
+    for (int j = 0; j < m.tz; j++) {
+      const uint32_t s = m.site(m.x, m.y, m.z0 + j);
+      double o[4];
+#pragma unroll
+      for (int a = 0; a < 4; a++) o[a] = in[a * V + s];
+      em_accumulate<kObsSums>(acc, o);
+    }
   }
-  __shared__ KSum sh[kPhibarThreads / 64];
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    KSum a = sh[0];
-    for (int i = 1; i < kPhibarThreads / 64; i++) ksum_merge(a, sh[i]);
-    phibar[n] = ksum_value(a) / (double)V;
-  }
+  em_store_slot<kObsSums>(acc, slots);
 }
+
+// The merge of the last launch of a run.
+__global__ __launch_bounds__(kEmBlock) void k_em_merge(EmMerge m, int64_t V) { em_merge(m, V); }
 
 // One adjoint step: lam_out = J(phi)^T (lam_in + dl e0), marching z with
 // lam_in's plane in LDS; accumulates the parameter gradients into the
@@ -346,8 +430,9 @@ struct lgv_em {
   double *ckpt = nullptr;   // nseg states
   double *seg = nullptr;    // K states of the segment being reversed
   double *work[2] = {nullptr, nullptr};
-  double *part = nullptr;   // per-block partial sums
+  double *part = nullptr;   // per-block sums of two consecutive launches
   double *phibar = nullptr; // nsteps
+  double *rows = nullptr;   // nsteps x kNobs: the measurement series
   GradSlot *gslots = nullptr;
   std::vector<double> phibar_host;
   bool have_forward = false;
@@ -355,32 +440,62 @@ struct lgv_em {
 
 namespace {
 
+// Step n: in -> out, its NS volume sums into slots, and block 0 merges
+// the slots of an earlier launch (prev).
+void launch_step(int NS, const lgv_em_params &p, uint64_t base, int nblocks, const double *in,
+                 double *out, int n, double *slots, const EmMerge &prev, hipStream_t st) {
+  const double gdt = p.gamma * p.dt;
+  const double amp = std::sqrt(2.0 * gdt);
+  const uint64_t skey = stream_key(base, kEmTag | (uint64_t)n);
+  if (NS == 0)
+    k_em_step<0><<<nblocks, kEmBlock, 0, st>>>(in, out, p.N, p.m2, p.lambda, p.H, gdt, amp,
+                                               skey, p.noise_kind, slots, prev);
+  else if (NS == 1)
+    k_em_step<1><<<nblocks, kEmBlock, 0, st>>>(in, out, p.N, p.m2, p.lambda, p.H, gdt, amp,
+                                               skey, p.noise_kind, slots, prev);
+  else
+    k_em_step<kObsSums><<<nblocks, kEmBlock, 0, st>>>(in, out, p.N, p.m2, p.lambda, p.H, gdt,
+                                                      amp, skey, p.noise_kind, slots, prev);
+}
+
 // Steps n0..n1-1 starting from `start`.  Forward mode (states == nullptr):
 // records phibar of every step, ping-pongs through the work buffers and
-// leaves phi_{n1} in `last`.  Recompute mode: states[i] = phi_{n0+i} for
-// i = 0..n1-n0 (start is copied to states[0]); the work buffers, which hold
+// leaves phi_{n1} in `last`; with every > 0 it also writes the observables
+// of the states every, 2 every, ... below nsteps into rows (row = state /
+// every).  `pending` (may be nullptr) is a merge an earlier launch left for
+// the first step.  Recompute mode: states[i] = phi_{n0+i} for i = 0..n1-n0
+// (start is copied to states[0]), no sums; the work buffers, which hold
 // the adjoint state during the backward pass, are not touched.
 int em_run_segment(lgv_em *h, const double *start, int n0, int n1, double *states,
-                   double *last) {
-  const double gdt = h->p.gamma * h->p.dt;
-  const double amp = std::sqrt(2.0 * gdt);
+                   double *last, int every = 0, double *rows = nullptr,
+                   const EmMerge *pending = nullptr) {
   const size_t st = 4 * h->V * sizeof(double);
   const double *in = start;
   if (states) {
     EMCHECK(hipMemcpyAsync(states, start, st, hipMemcpyDeviceToDevice, h->stream));
     in = states;
   }
+  EmMerge prev{nullptr, 1, h->nblocks, nullptr, nullptr};
+  if (pending) prev = *pending;
   for (int n = n0; n < n1; n++) {
     double *out = states ? states + (int64_t)(n - n0 + 1) * 4 * h->V
                          : ((n + 1 == n1) ? last : h->work[(n - n0) & 1]);
-    const uint64_t skey = stream_key(h->base, kEmTag | (uint64_t)n);
-    k_em_step<<<h->nblocks, kEmBlock, 0, h->stream>>>(in, out, h->p.N, h->p.m2, h->p.lambda,
-                                                      h->p.H, gdt, amp, skey, h->p.noise_kind,
-                                                      h->part);
-    if (!states)  // the reverse pass uses the forward's phibar
-      k_em_phibar<<<1, kPhibarThreads, 0, h->stream>>>(h->part, h->nblocks, h->V, h->phibar, n);
+    double *slots = h->part + (size_t)(n & 1) * h->nblocks * kPartStride;
+    EmMerge mine{slots, 1, h->nblocks, nullptr, nullptr};
+    int NS = 0;
+    if (!states) {  // the reverse pass uses the forward's phibar
+      NS = 1;
+      mine.phibar = h->phibar + n;
+      if (every > 0 && (n + 1) % every == 0 && n + 1 < h->nsteps) {
+        NS = mine.ns = kObsSums;
+        mine.obs = rows + (size_t)((n + 1) / every) * kNobs;
+      }
+    }
+    launch_step(NS, h->p, h->base, h->nblocks, in, out, n, slots, prev, h->stream);
+    prev = NS ? mine : EmMerge{nullptr, 1, h->nblocks, nullptr, nullptr};
     in = out;
   }
+  if (prev.slots) k_em_merge<<<1, kEmBlock, 0, h->stream>>>(prev, h->V);
   if (!states && n1 == n0 && last != start)
     EMCHECK(hipMemcpyAsync(last, start, st, hipMemcpyDeviceToDevice, h->stream));
   EMCHECK(hipGetLastError());
@@ -425,8 +540,12 @@ int lgv_em_create(const lgv_em_params *p, int nsteps, int ckpt_every, int device
   if ((e = hipMalloc(&h->seg, st * h->K)) != hipSuccess) return cleanup_fail(e);
   for (int i = 0; i < 2; i++)
     if ((e = hipMalloc(&h->work[i], st)) != hipSuccess) return cleanup_fail(e);
-  if ((e = hipMalloc(&h->part, 2 * h->nblocks * sizeof(double))) != hipSuccess) return cleanup_fail(e);
+  if ((e = hipMalloc(&h->part, 2 * (size_t)h->nblocks * kPartStride * sizeof(double))) !=
+      hipSuccess)
+    return cleanup_fail(e);
   if ((e = hipMalloc(&h->phibar, nsteps * sizeof(double))) != hipSuccess) return cleanup_fail(e);
+  if ((e = hipMalloc(&h->rows, (size_t)nsteps * kNobs * sizeof(double))) != hipSuccess)
+    return cleanup_fail(e);
   if ((e = hipMalloc(&h->gslots, h->nblocks * sizeof(GradSlot))) != hipSuccess) return cleanup_fail(e);
   *out = h;
   return LGV_OK;
@@ -537,6 +656,44 @@ int lgv_em_backward(lgv_em *h, double *grad_phi0, double grad_params[3]) {
   return LGV_OK;
 }
 
+int lgv_em_series(lgv_em *h, const double *phi0, int every, double *phi_final,
+                  double *rows_host) {
+  if (!h || !phi0 || !rows_host) return em_fail(LGV_ERR_ARG, "null argument");
+  if (every <= 0) return em_fail(LGV_ERR_ARG, "every must be positive");
+  EMCHECK(hipSetDevice(h->device));
+  const int nrows = (h->nsteps + every - 1) / every;
+  // the measurement before the first step, then one after every `every`
+  // steps, inside the step launch that produces the state
+  // (slots of parity 1, merged by step 0's launch)
+  double *slots = h->part + (size_t)h->nblocks * kPartStride;
+  k_em_observe<<<h->nblocks, kEmBlock, 0, h->stream>>>(phi0, h->p.N, slots);
+  EMCHECK(hipGetLastError());
+  const EmMerge row0{slots, kObsSums, h->nblocks, nullptr, h->rows};
+  int rc;
+  if ((rc = em_run_segment(h, phi0, 0, h->nsteps, nullptr, phi_final ? phi_final : h->seg,
+                           every, h->rows, &row0)))
+    return rc;
+  EMCHECK(hipMemcpyAsync(rows_host, h->rows, (size_t)nrows * kNobs * sizeof(double),
+                         hipMemcpyDeviceToHost, h->stream));
+  EMCHECK(hipStreamSynchronize(h->stream));
+  return LGV_OK;
+}
+
+int lgv_em_step(lgv_em *h, const double *in, double *out, int n, double *obs) {
+  if (!h || !in || !out) return em_fail(LGV_ERR_ARG, "null argument");
+  if (n < 0) return em_fail(LGV_ERR_ARG, "negative step index");
+  if (in == out) return em_fail(LGV_ERR_ARG, "in and out must differ");
+  EMCHECK(hipSetDevice(h->device));
+  const EmMerge none{nullptr, 1, h->nblocks, nullptr, nullptr};
+  launch_step(obs ? kObsSums : 0, h->p, h->base, h->nblocks, in, out, n, h->part, none,
+              h->stream);
+  if (obs)
+    k_em_merge<<<1, kEmBlock, 0, h->stream>>>(EmMerge{h->part, kObsSums, h->nblocks, nullptr, obs},
+                                              h->V);
+  EMCHECK(hipGetLastError());
+  return LGV_OK;
+}
+
 int lgv_em_destroy(lgv_em *h) {
   if (!h) return LGV_OK;
   // releases everything even after a failure and reports the first one
@@ -547,7 +704,7 @@ int lgv_em_destroy(lgv_em *h) {
   note(hipSetDevice(h->device));
   if (h->stream) note(hipStreamSynchronize(h->stream));
   for (void *q : {(void *)h->ckpt, (void *)h->seg, (void *)h->work[0], (void *)h->work[1],
-                  (void *)h->part, (void *)h->phibar, (void *)h->gslots})
+                  (void *)h->part, (void *)h->phibar, (void *)h->rows, (void *)h->gslots})
     note(hipFree(q));
   if (h->own_stream && h->stream) note(hipStreamDestroy(h->stream));
   delete h;

@@ -9,6 +9,7 @@ from .lgv import LgvError, lib
 
 NOISE_UNIFORM = 0   # variance1: U(-sqrt3, sqrt3), NoiseGenerator.h:105
 NOISE_GAUSSIAN = 1  # Box-Muller from two canonical draws
+NOBS = 6  # phibar_0..3, <|phi|^2>, V |phibar|^2 (LGV_EM_NOBS)
 
 
 class EmParams(C.Structure):
@@ -33,6 +34,8 @@ def _bind():
                                  C.POINTER(C.c_double)]
     L.lgv_em_backward.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]
     L.lgv_em_destroy.argtypes = [C.c_void_p]
+    L.lgv_em_series.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    L.lgv_em_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
     _bound = True
     return L
 
@@ -64,6 +67,21 @@ class EulerMaruyama:
                                       phibar.ctypes.data_as(C.c_void_p), C.byref(loss)))
         return loss.value, phibar
 
+    def series(self, phi0_ptr, every, phi_final_ptr=None):
+        """Observables (phibar_0..3, <|phi|^2>, V |phibar|^2) of the states
+        0, every, 2 every, ... below nsteps: an array (ceil(nsteps/every), 6)."""
+        rows = np.zeros(((self.nsteps + every - 1) // every, NOBS))
+        _check(_bind().lgv_em_series(self.h, C.c_void_p(phi0_ptr), every,
+                                     C.c_void_p(phi_final_ptr) if phi_final_ptr else None,
+                                     rows.ctypes.data_as(C.c_void_p)))
+        return rows
+
+    def step(self, in_ptr, out_ptr, n, obs_ptr=None):
+        """Step n from in to out (device addresses), stream-ordered; obs_ptr
+        (device, 6 doubles) receives the observables of out."""
+        _check(_bind().lgv_em_step(self.h, C.c_void_p(in_ptr), C.c_void_p(out_ptr), n,
+                                   C.c_void_p(obs_ptr) if obs_ptr else None))
+
     def backward(self, grad_phi0_ptr):
         gp = (C.c_double * 3)()
         _check(_bind().lgv_em_backward(self.h, C.c_void_p(grad_phi0_ptr), gp))
@@ -79,3 +97,10 @@ class EulerMaruyama:
             self.close()
         except Exception:
             pass
+
+
+def series_susceptibility(rows, V):
+    """V (<|phibar|^2> - <|phibar|>^2) over a measurement series: the O(4)
+    order-parameter susceptibility from the rows of EulerMaruyama.series."""
+    mag = np.sqrt((rows[:, :4] ** 2).sum(axis=1))
+    return V * ((mag ** 2).mean() - mag.mean() ** 2)

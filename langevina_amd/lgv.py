@@ -60,7 +60,7 @@ EXPORTS = [
     "lgv_field_device_ptr", "lgv_timing_enable", "lgv_timing_get", "lgv_measure_full",
     "lgv_set_mass", "lgv_monitor_reset", "lgv_set_ideal_tile",
     "lgv_em_last_error", "lgv_em_create", "lgv_em_set_stream", "lgv_em_forward",
-    "lgv_em_backward", "lgv_em_destroy",
+    "lgv_em_backward", "lgv_em_destroy", "lgv_em_series", "lgv_em_step",
 ]
 
 
@@ -131,6 +131,34 @@ def model_params(NX, LX=0, mass0=-4.70052, dmassdt=0.0, lam=4.0, H=0.003, chi=5.
     p.diffusion = 1.0 / 3.0 * gamma if diffusion is None else diffusion
     p.seed, p.initialtime = seed, initialtime
     return p
+
+
+def _xoro_uniforms(seed, n):
+    """n uniform() draws (NoiseGenerator::uniform, NoiseGenerator.h:68-69) of
+    a xoroshiro128+ engine seeded by xoroshiro128plus::seed(seed)
+    (xoroshiro128plus.h:86-107): generate_canonical of one 64-bit output."""
+    m = 2 ** 64
+    s0 = (0x01d353e5f3993bb0 + seed) % m
+    s1 = (0x7b9c0df6cb193b20 * (seed + 1)) % m
+    out = []
+    for _ in range(n):
+        res = (s0 + s1) % m
+        t = s1 ^ s0
+        s0 = (((s0 << 55) | (s0 >> 9)) % m) ^ t ^ ((t << 14) % m)
+        s1 = ((t << 36) | (t >> 28)) % m
+        u = float(res) / m
+        out.append(u if u < 1.0 else float.fromhex("0x1.fffffffffffffp-1"))
+    return out
+
+
+def ensemble_params(sizes, seed0=1000, H_range=(0.003, 0.03), H_seed=12345, **par):
+    """Parameters of a batch of independent replicas: replica r has side
+    sizes[r], seed seed0 + r and H uniform in H_range, drawn from one
+    xoroshiro128+ stream seeded with H_seed; par sets the other couplings."""
+    lo, hi = H_range
+    us = _xoro_uniforms(H_seed, len(sizes))
+    return [model_params(N, seed=seed0 + r, **dict(par, H=lo + (hi - lo) * us[r]))
+            for r, N in enumerate(sizes)]
 
 
 class Context:

@@ -971,6 +971,39 @@ double orc_em_run(const orc_em_params *p, const double *phi0, int nsteps, int wi
   return (double)(m + (long double)V * (m2 - m * m));
 }
 
+/* The observables of one state (phi[V][4]): the volume means phibar_0..3,
+ * the volume mean of |phi|^2 and V |phibar|^2 -- the volume averages a
+ * measurement takes (Measurer, measurer.h:169-179), summed in long double. */
+static void em_observe(size_t V, const double *phi, double *o) {
+  long double s[5] = {0, 0, 0, 0, 0};
+  for (size_t x = 0; x < V; x++) {
+    const double *c = phi + 4 * x;
+    for (int a = 0; a < 4; a++) s[a] += c[a];
+    s[4] += ((c[0] * c[0] + c[1] * c[1]) + c[2] * c[2]) + c[3] * c[3];
+  }
+  for (int i = 0; i < 5; i++) o[i] = (double)(s[i] / (long double)V);
+  o[5] = (double)V * (((o[0] * o[0] + o[1] * o[1]) + o[2] * o[2]) + o[3] * o[3]);
+}
+
+/* The run loop of SuperPions.cxx:124-160 (measure, then step) on the EM
+ * stepper: rows[r] = observables of the state after r*every steps, for
+ * r*every < nsteps. */
+void orc_em_series(const orc_em_params *p, const double *phi0, int nsteps, int every,
+                   double *rows /*[ceil(nsteps/every)][6]*/, double *phi_final) {
+  const size_t V = (size_t)p->N * p->N * p->N;
+  double *a = (double *)malloc(V * 4 * sizeof(double));
+  double *b = (double *)malloc(V * 4 * sizeof(double));
+  memcpy(a, phi0, V * 4 * sizeof(double));
+  for (int n = 0; n < nsteps; n++) {
+    if (n % every == 0) em_observe(V, a, rows + (size_t)(n / every) * 6);
+    orc_em_step(p, (uint64_t)n, a, b);
+    double *t = a; a = b; b = t;
+  }
+  if (phi_final) memcpy(phi_final, a, V * 4 * sizeof(double));
+  free(a);
+  free(b);
+}
+
 /* Reverse-mode derivative of orc_em_run with the noise held fixed: the
  * Jacobian of one step is symmetric, J = I - G dt (-Lap + m2 + lambda
  * (|phi|^2 + 2 phi phi^T)), so the adjoint step applies the same stencil.

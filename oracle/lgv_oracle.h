@@ -132,6 +132,8 @@ void orc_em_step(const orc_em_params *p, uint64_t step, const double *phi,
                  double *out /*[N^3][4]*/);
 double orc_em_run(const orc_em_params *p, const double *phi0, int nsteps,
                   int window, double *phibar /*[nsteps]*/, double *phi_final);
+void orc_em_series(const orc_em_params *p, const double *phi0, int nsteps, int every,
+                   double *rows /*[ceil(nsteps/every)][6]*/, double *phi_final);
 void orc_em_adjoint(const orc_em_params *p, const double *phi0, int nsteps,
                     int window, double *grad_phi0, double *grad_params /*[3] m2,H,lambda*/);
 

@@ -135,6 +135,7 @@ def lib():
         L.orc_em_run.argtypes = [ep, dp, C.c_int, C.c_int, dp, C.c_void_p]
         L.orc_em_run.restype = C.c_double
         L.orc_em_adjoint.argtypes = [ep, dp, C.c_int, C.c_int, dp, dp]
+        L.orc_em_series.argtypes = [ep, dp, C.c_int, C.c_int, dp, C.c_void_p]
         _lib = L
     return _lib
 
@@ -388,6 +389,16 @@ def em_run(p, phi0, nsteps, window):
     loss = lib().orc_em_run(C.byref(p), np.ascontiguousarray(phi0).reshape(-1), nsteps,
                             window, phibar, fin.ctypes.data_as(C.c_void_p))
     return loss, phibar, fin
+
+
+def em_series(p, phi0, nsteps, every):
+    """rows[r] = (phibar_0..3, <|phi|^2>, V |phibar|^2) of the state after
+    r*every steps (r*every < nsteps), and the final state."""
+    rows = np.zeros(((nsteps + every - 1) // every, 6))
+    fin = np.zeros_like(phi0)
+    lib().orc_em_series(C.byref(p), np.ascontiguousarray(phi0).reshape(-1), nsteps, every,
+                        rows.reshape(-1), fin.ctypes.data_as(C.c_void_p))
+    return rows, fin
 
 
 def em_adjoint(p, phi0, nsteps, window):

@@ -84,3 +84,45 @@ def test_adjoint_against_finite_differences(orc):
     v = np.random.default_rng(5).normal(size=phi0.shape)
     fd = (loss(params(), phi0 + eps * v) - loss(params(), phi0 - eps * v)) / (2 * eps)
     assert abs((g * v).sum() - fd) <= 1e-6 * abs(fd)
+
+
+def test_em_series_matches_numpy_means(orc):
+    # measure-then-step: row r holds the state after r*every steps
+    N, nsteps, every = 6, 23, 5
+    p = orc.em_params(N, -0.6, 0.9, 0.05, 1.0, 0.03, 9, 1)
+    phi = np.random.default_rng(4).normal(size=(N, N, N, 4)) * 0.2
+    rows, fin = orc.em_series(p, phi, nsteps, every)
+    assert rows.shape == (5, 6)
+    V = N ** 3
+    for n in range(nsteps):
+        if n % every == 0:
+            want = list(phi.reshape(-1, 4).mean(axis=0))
+            want.append((phi ** 2).sum(axis=-1).mean())
+            want.append(V * sum(w * w for w in want[:4]))
+            np.testing.assert_allclose(rows[n // every], want, rtol=1e-13, atol=1e-16)
+        phi = orc.em_step(p, n, phi)
+    np.testing.assert_array_equal(fin, phi)
+    phi0 = np.random.default_rng(4).normal(size=(N, N, N, 4)) * 0.2
+    np.testing.assert_array_equal(fin, orc.em_run(p, phi0, nsteps, 1)[2])
+
+
+def test_em_series_free_field_site_variance(orc):
+    # free field: every Fourier mode of forward Euler has the stationary
+    # variance 2 / (w_k^2 (2 - G dt w_k^2)), so the mean of |phi|^2 over the
+    # lattice is 4/V sum_k of it (four components) and the k = 0 mode gives
+    # V <phibar_a^2> = 2 / (m^2 (2 - G dt m^2))  (freeCase.ipynb free2ptFE)
+    N, m2, dt = 6, 1.0, 0.1
+    p = orc.em_params(N, m2, 0.0, 0.0, 1.0, dt, 13, 1)
+    rows, _ = orc.em_series(p, np.zeros((N, N, N, 4)), 30000, 3)
+    rows = rows[100:]
+    k = 2 * np.pi * np.arange(N) / N
+    w2 = m2 + sum(np.meshgrid(*(2 - 2 * np.cos(k),) * 3, indexing="ij"))
+    site = 4 * (2 / (w2 * (2 - dt * w2))).mean()
+    blocks = rows[:, 4].reshape(20, -1).mean(axis=1)
+    err = blocks.std() / math.sqrt(20)
+    assert abs(blocks.mean() - site) < 4 * err + 1e-3 * site, (blocks.mean(), site, err)
+    V = N ** 3
+    zero = 2 / (m2 * (2 - dt * m2))
+    b0 = (V * rows[:, :4] ** 2).mean(axis=1).reshape(20, -1).mean(axis=1)
+    e0 = b0.std() / math.sqrt(20)
+    assert abs(b0.mean() - zero) < 4 * e0 + 0.01 * zero, (b0.mean(), zero, e0)
