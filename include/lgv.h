@@ -222,6 +222,18 @@ int lgv_em_series(lgv_em *h, const double *phi0, int every, double *phi_final,
  * synchronisation.  obs (device memory, may be NULL) receives the
  * LGV_EM_NOBS observables of out. */
 int lgv_em_step(lgv_em *h, const double *in, double *out, int n, double *obs);
+/* lgv_em_step without a handle and without allocating anything: for
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  No counterpart in the reference.  The
+ * step's reduction slots live in one static device array per GPU (640 KiB,
+ * held until the process ends), so N <= 256 (LGV_ERR_RANGE beyond);
+ * calls that overlap on one GPU (two streams, two threads) race on it.
+ * Same arguments and results as lgv_em_step with the handle's parameters,
+ * on `device` (made current) and `hip_stream` (NULL = the null stream).
+ * The first call on a device looks the array's address up (no stream
+ * work); make that call before a graph capture. */
+int lgv_em_step_noalloc(const lgv_em_params *p, const double *in, double *out, int n,
+                        double *obs, int device, void *hip_stream);
 int lgv_em_destroy(lgv_em *h);
 
 #ifdef __cplusplus

@@ -18,6 +18,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -502,6 +503,49 @@ int em_run_segment(lgv_em *h, const double *start, int n0, int n1, double *state
   return LGV_OK;
 }
 
+// std::seed_seq{v}.generate() of one word ([rand.util.seedseq], libstdc++ 11
+// bits/random.tcc:3250-3330) without the heap: with one input and one
+// output word every index of the algorithm is 0, the first loop leaves
+// b = 1664525 T(b0) + 1 + ... , and the result is
+// 1566083941 T(3 (1664525 T(r) + v)), T(x) = x ^ (x >> 27), r = 1664525
+// T(0x8b8b8b8b) + 1, all in 32-bit arithmetic.
+uint32_t seed_word(uint32_t v) {
+  auto T = [](uint32_t x) { return x ^ (x >> 27); };
+  const uint32_t r0 = 1664525u * T(0x8b8b8b8bu) + 1u;
+  const uint32_t b1 = 1664525u * T(r0) + v;
+  return 1566083941u * T(3u * b1);
+}
+
+// ---- the entry point that allocates nothing (lgv_em_step_noalloc)
+//
+// Its reduction slots live in one file-scope device array per GPU, sized
+// for em_blocks(N) <= kScratchBlocks (every N <= 256).  Host code
+// never passes `scratch` itself to a kernel (on the host it names a
+// stand-in, not device memory): the device address is looked up once per
+// device with hipGetSymbolAddress and kept in g_scratch.
+constexpr int kScratchBlocks = 8192;  // em_blocks(256) = 4 x 64 x 32
+constexpr size_t kScratchDoubles = (size_t)kScratchBlocks * kPartStride;  // 640 KiB
+}  // namespace
+static __device__ __align__(256) double scratch[kScratchDoubles];
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices];
+
+// The device address of `scratch` on `device` (current), looked up on the
+// first call for that device.  Nothing in it needs clearing: every launch
+// writes the slots it merges.
+int scratch_on(int device, double **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void *ptr = nullptr;
+    EMCHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch[device] = static_cast<double *>(ptr);
+  }
+  *out = g_scratch[device];
+  return LGV_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -690,6 +734,34 @@ int lgv_em_step(lgv_em *h, const double *in, double *out, int n, double *obs) {
   if (obs)
     k_em_merge<<<1, kEmBlock, 0, h->stream>>>(EmMerge{h->part, kObsSums, h->nblocks, nullptr, obs},
                                               h->V);
+  EMCHECK(hipGetLastError());
+  return LGV_OK;
+}
+
+int lgv_em_step_noalloc(const lgv_em_params *p, const double *in, double *out, int n,
+                        double *obs, int device, void *stream) {
+  if (!p || !in || !out) return em_fail(LGV_ERR_ARG, "null argument");
+  if (in == out) return em_fail(LGV_ERR_ARG, "in and out must differ");
+  if (p->N < 4 || p->N > 256 || em_blocks(p->N) > kScratchBlocks)
+    return em_fail(LGV_ERR_RANGE, "lgv_em_step_noalloc: N=%d outside [4,256]", p->N);
+  if (p->noise_kind != 0 && p->noise_kind != 1)
+    return em_fail(LGV_ERR_ARG, "noise_kind must be 0 or 1");
+  if (n < 0) return em_fail(LGV_ERR_ARG, "negative step index");
+  if (device < 0 || device >= kMaxDevices)
+    return em_fail(LGV_ERR_ARG, "device %d outside [0,%d)", device, kMaxDevices);
+  EMCHECK(hipSetDevice(device));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  double *d = nullptr;
+  int rc;
+  if ((rc = scratch_on(device, &d))) return rc;
+  const int nblocks = em_blocks(p->N);
+  const EmMerge none{nullptr, 1, nblocks, nullptr, nullptr};
+  launch_step(obs ? kObsSums : 0, *p, base_key(seed_word((uint32_t)p->seed)), nblocks, in, out,
+              n, d, none, st);
+  if (obs) {
+    const int64_t V = (int64_t)p->N * p->N * p->N;
+    k_em_merge<<<1, kEmBlock, 0, st>>>(EmMerge{d, kObsSums, nblocks, nullptr, obs}, V);
+  }
   EMCHECK(hipGetLastError());
   return LGV_OK;
 }

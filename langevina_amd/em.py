@@ -36,6 +36,8 @@ def _bind():
     L.lgv_em_destroy.argtypes = [C.c_void_p]
     L.lgv_em_series.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
     L.lgv_em_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+    L.lgv_em_step_noalloc.argtypes = [C.POINTER(EmParams), C.c_void_p, C.c_void_p, C.c_int,
+                                      C.c_void_p, C.c_int, C.c_void_p]
     _bound = True
     return L
 
@@ -97,6 +99,16 @@ class EulerMaruyama:
             self.close()
         except Exception:
             pass
+
+
+def step_noalloc(N, m2, lam, H, gamma, dt, seed, in_ptr, out_ptr, n, obs_ptr=None,
+                 noise_kind=NOISE_UNIFORM, device=0, stream=None):
+    """lgv_em_step_noalloc: one step without a handle and without any
+    device allocation (N <= 256; calls overlapping on one GPU race)."""
+    p = EmParams(N, m2, lam, H, gamma, dt, seed, noise_kind)
+    _check(_bind().lgv_em_step_noalloc(C.byref(p), C.c_void_p(in_ptr), C.c_void_p(out_ptr), n,
+                                       C.c_void_p(obs_ptr) if obs_ptr else None, device,
+                                       C.c_void_p(stream) if stream else None))
 
 
 def series_susceptibility(rows, V):

@@ -61,6 +61,7 @@ EXPORTS = [
     "lgv_set_mass", "lgv_monitor_reset", "lgv_set_ideal_tile",
     "lgv_em_last_error", "lgv_em_create", "lgv_em_set_stream", "lgv_em_forward",
     "lgv_em_backward", "lgv_em_destroy", "lgv_em_series", "lgv_em_step",
+    "lgv_em_step_noalloc",
 ]
 
 
