@@ -2,14 +2,18 @@
 """Throughput of the ModelG time step (PV2HBSplit23) on one MI355X.
 
     python bench.py [--steps K] [--warmup W] [--size 256] [--precision fp64|fp32]
-                    [--graph] [--batched] [--drift] [--em] [--no-cpu-baseline]
-                    [--calibrate-copy] [--no-autotune | --ideal-tile CAP TZ]
+                    [--graph] [--headline-only] [--batched] [--drift] [--em]
+                    [--no-cpu-baseline] [--calibrate-copy]
+                    [--no-autotune | --ideal-tile CAP TZ]
 
 One "step" is PV2HBSplit::step(dt) (3 ideal substeps, 12 phi Metropolis
 half-sweeps, 6 charge sub-sweeps; here 3 + 12 + 3 kernel launches) applied to
-the whole lattice; the metric
-is site-updates/s = sites x steps / wall seconds, inputs resident in HBM.
-Prints one JSON line.
+the whole lattice; the metric is site-updates/s = sites x steps / wall
+seconds, inputs resident in HBM.  The default run reports the 256^3 FP64
+lattice as `value`, and beside it the FP32-field form, the 96-replica
+batched FP32 ensemble (and its one-at-a-time figure), the FP32-vs-FP64
+observable drift over paired seeds, and the CPU oracle baselines.  Prints
+one JSON line.
 """
 import argparse
 import json
@@ -100,38 +104,67 @@ PARAMS = dict(mass0=-4.813, lam=1.0, H=0.01, chi=2.0, gamma=1.0)
 DT = 0.04
 
 
-def cpu_baseline(seconds_target=12.0):
-    """The CPU oracle (a C restatement of the reference's stepper) timed on
-    this host on a bounded sample: 128^3 FP64 PV2HBSplit23 steps until
-    ~seconds_target of work, its loops split over z-planes across the host's
-    cores (at most 16, the CPU share of one GPU), as MPI ranks split the
-    reference's lattice."""
-    from oracle import oracle as orc
-    N = 128
-    cores = max(1, min(16, os.cpu_count() or 1))
-    orc.set_threads(cores)
-    lat = orc.Lattice(N, mass0=PARAMS["mass0"], lam=PARAMS["lam"], H=PARAMS["H"],
-                      chi=PARAMS["chi"], seed=42, rng_mode=orc.RNG_PER_SITE)
-    lat.init_gaussian_charges()
-    lat.step(DT)  # first touch of the pages and thread start-up
-    steps, t0 = 0, time.perf_counter()
-    while True:
-        lat.step(DT)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el > seconds_target or steps >= 40:
-            break
-    orc.set_threads(1)
+def host_cpu():
+    """(model name, nproc) of this host."""
     model = ""
     try:
         with open("/proc/cpuinfo") as f:
             model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
     except OSError:
         pass
+    return model, os.cpu_count()
+
+
+def _time_oracle(lat, seconds, max_steps):
+    lat.step(DT)  # first touch of the pages and thread start-up
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        lat.step(DT)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > seconds or steps >= max_steps:
+            return steps, el
+
+
+def cpu_baseline(seconds_all=10.0, seconds_one=8.0):
+    """The CPU oracle (a C restatement of the reference's stepper) timed on
+    this host on bounded samples:
+      * all cores (at most 16, the CPU share of one GPU), per-site streams,
+        the headline 256^3 FP64 lattice: its loops split over z-planes, as
+        MPI ranks split the reference's lattice;
+      * one core in the reference's own RNG order (one global engine, the
+        sites in k, j, i order: the single-rank reference trajectory) on a
+        64^3 lattice -- the stand-in for the reference's CPU build, whose
+        PETSc and FFTW dependencies are absent here."""
+    from oracle import oracle as orc
+    model, nproc = host_cpu()
+    cores = max(1, min(16, nproc or 1))
+    kw = dict(mass0=PARAMS["mass0"], lam=PARAMS["lam"], H=PARAMS["H"], chi=PARAMS["chi"],
+              seed=42)
+    orc.set_threads(cores)
+    N = 256
+    lat = orc.Lattice(N, rng_mode=orc.RNG_PER_SITE, **kw)
+    lat.init_gaussian_charges()
+    steps, el = _time_oracle(lat, seconds_all, 20)
+    del lat
+    orc.set_threads(1)
+    n1 = 64
+    lat = orc.Lattice(n1, rng_mode=orc.RNG_REFERENCE, **kw)
+    lat.init_gaussian_charges()
+    s1, e1 = _time_oracle(lat, seconds_one, 100)
+    del lat
+    host = f"{model}, nproc {nproc}"
     return {"value": N ** 3 * steps / el, "unit": "site-updates/s", "cores": cores,
-            "kind": "port", "host": f"{model}, nproc {os.cpu_count()}",
-            "sample": f"oracle/lgv_oracle.c, {steps} PV2HBSplit23 step(s) of a {N}^3 FP64 "
-                      f"lattice after one untimed step ({el:.1f} s, {cores} OpenMP threads)"}
+            "kind": "port", "host": host,
+            "sample": f"oracle/lgv_oracle.c per-site-stream mode, {steps} PV2HBSplit23 step(s) "
+                      f"of a {N}^3 FP64 lattice after one untimed step ({el:.1f} s, {cores} "
+                      f"OpenMP threads over z-planes)",
+            "single_thread_reference_order": {
+                "value": n1 ** 3 * s1 / e1, "unit": "site-updates/s", "cores": 1,
+                "kind": "port", "host": host,
+                "sample": f"oracle/lgv_oracle.c reference RNG order (one engine, k-j-i site "
+                          f"order), {s1} step(s) of a {n1}^3 FP64 lattice after one untimed "
+                          f"step ({e1:.1f} s, 1 thread)"}}
 
 
 def batched_replicas(L):
@@ -196,119 +229,222 @@ def bench_em(args, N=128, nsteps=500, ckpt=50, window=250, reps=3):
     print(json.dumps(out), flush=True)
 
 
-def bench_batched(args):
-    import langevina_amd as L
-    from langevina_amd.autotune import autotune_ideal
+METRIC = "site-updates/sec (ModelG 256^3 FP64 vs FP32/batched) and % HBM roofline, 1 GPU"
+T_START = time.perf_counter()
+
+
+def progress(what):
+    """One line per finished part on stderr (stdout carries only the JSON)."""
+    print(f"bench: {what} ({time.perf_counter() - T_START:.1f} s)", file=sys.stderr, flush=True)
+
+
+def kernel_table(tm, alg, kern, sites, esz):
+    """Per kernel class: total ms, launches, average launch and algorithmic
+    GB/s from the HIP-event timing of a run."""
+    return {k: {"kernel": kern[k], "ms_total": round(v[0], 3), "launches": v[1],
+                "avg_us": round(v[0] * 1e3 / max(v[1], 1), 2),
+                "alg_GBps": round(alg[k] * sites * esz / 8 /
+                                  (v[0] * 1e-3 / max(v[1], 1)) / 1e9, 1) if v[1] else None}
+            for k, v in tm.items() if k in alg}
+
+
+def roofline_of(tm, alg, kern, sites, esz, precision, N=None):
+    """The roofline object of the dominant kernel class (largest total time)."""
+    dom = max(("ideal", "phi", "charge"), key=lambda k: tm[k][0])
+    ms, launches = tm[dom]
+    per_launch_s = ms * 1e-3 / max(launches, 1)
+    bytes_per_launch = alg[dom] * sites * esz // 8
+    achieved = bytes_per_launch / per_launch_s
+    return {"bound": "hbm", "kernel": kern[dom], "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
+            "traffic": traffic(kern[dom], precision, N) if N else None,
+            "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": per_launch_s * 1e6}
+
+
+def run_single(L, N, fp32, steps, warmup, graph=False, tune=True, ideal_tile=None,
+               measure=False):
+    """One N^3 lattice (phi = 0, Gaussian charges): warmup steps, then steps
+    timed with a host clock around a synchronised run and with HIP events per
+    kernel class.  Returns the measured figures."""
+    prec = L.FP32 if fp32 else L.FP64
+    esz = 4 if fp32 else 8
+    ctx = L.Context(L.model_params(N, seed=42, **PARAMS), precision=prec)
+    # synthetic start: phi = 0 (ModelA::initialize), Gaussian charges with the
+    # zero mode removed (ModelA::initialize_gaussian_charges)
+    ctx.init_gaussian_charges()
+    tuned = {}
+    if ideal_tile:
+        from langevina_amd.autotune import set_ideal_tile
+        set_ideal_tile(ctx, 0, *ideal_tile)
+        tuned = {N: tuple(ideal_tile)}
+    elif tune:
+        # ideal-kernel tile for this lattice size; the tuning steps are warmup
+        from langevina_amd.autotune import autotune_ideal
+        tuned = autotune_ideal(ctx, DT)
+    if graph:
+        ctx.use_graph(True)
+    ctx.run(DT, warmup)
+    ctx.synchronize()
+    ctx.timing_enable(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.run(DT, steps)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    tm = ctx.timing()
+    ctx.timing_enable(False)
+    sites = N ** 3
+    value = sites * steps / el
+    alg, kern, sched = tables(fp32, sites * 10 * esz)
+    out = {
+        "value": value, "ms_per_step": el * 1e3 / steps, "steps": steps, "warmup": warmup,
+        "dtype": "f32-storage/f64-arith" if fp32 else "f64",
+        "workload": f"ModelG {N}^3 PV2HBSplit23 {'FP32 fields' if fp32 else 'FP64'}",
+        "ideal_tile": list(tuned[N][:2]) if tuned else "default (32, 16)",
+        "step_roofline_frac": value * ALG_BYTES_PER_SITE_UPDATE * esz / 8 / HBM_PEAK,
+        "step_schedule_bytes_frac": value * sched * esz / 8 / HBM_PEAK,
+        "roofline": roofline_of(tm, alg, kern, sites, esz, "fp32" if fp32 else "fp64", N),
+        "kernels": kernel_table(tm, alg, kern, sites, esz),
+    }
+    if measure:
+        # the measurement pass (Measurer::measure), reported separately
+        ctx.measure_full()
+        tm0 = time.perf_counter()
+        for _ in range(5):
+            ctx.measure_full()
+        meas_s = (time.perf_counter() - tm0) / 5
+        out["measurement"] = {"ms": meas_s * 1e3, "alg_GBps": sites * 10 * esz / meas_s / 1e9,
+                              "what": "lgv_measure_full: walls, phase walls, averages, device "
+                                      "r2c of every row, rotate_to_vev; host copies included"}
+    ctx.close()
+    return out
+
+
+def run_batched(L, fp32, steps, warmup, tune=True):
+    """The 96-replica ensemble in one context, one HIP graph per step, against
+    the same replicas one context (and one graph) at a time."""
+    from langevina_amd.autotune import autotune_ideal, set_ideal_tile
     reps = batched_replicas(L)
     sites = sum(p.NX ** 3 for p in reps)
-    prec = L.FP32 if args.precision == "fp32" else L.FP64
+    prec = L.FP32 if fp32 else L.FP64
+    esz = 4 if fp32 else 8
     ctx = L.Context(reps, precision=prec)
     for r in range(len(reps)):
         ctx.init_gaussian_charges(r)
-    tuned = autotune_ideal(ctx, DT) if not args.no_autotune else {}
+    tuned = autotune_ideal(ctx, DT) if tune else {}
     ctx.timing_enable(True)  # kernel breakdown from a short direct-launch pass
     ctx.run(DT, 5)
     tm = ctx.timing()
     ctx.timing_enable(False)
     ctx.use_graph(True)
-    ctx.run(DT, args.warmup)
+    ctx.run(DT, warmup)
     ctx.synchronize()
     t0 = time.perf_counter()
-    ctx.run(DT, args.steps)
+    ctx.run(DT, steps)
     ctx.synchronize()
     el = time.perf_counter() - t0
-    value = sites * args.steps / el
+    value = sites * steps / el
     ctx.close()
-    # the same replicas one context (and one graph) at a time
     seq_el = 0.0
     for p in reps:
         c1 = L.Context(p, precision=prec)
         c1.init_gaussian_charges()
         if tuned:
-            from langevina_amd.autotune import set_ideal_tile
             cap, tz = tuned[p.NX][:2]
             set_ideal_tile(c1, 0, cap, tz)
         c1.use_graph(True)
         c1.run(DT, 2)
         c1.synchronize()
         t0 = time.perf_counter()
-        c1.run(DT, args.steps)
+        c1.run(DT, steps)
         c1.synchronize()
         seq_el += time.perf_counter() - t0
         c1.close()
-    seq_value = sites * args.steps / seq_el
-    esz = 8 if prec == L.FP64 else 4
-    alg, kern, _ = tables(prec == L.FP32, sites * 10 * esz)
-    dom = max(("ideal", "phi", "charge"), key=lambda k: tm[k][0])
-    ms, launches = tm[dom]
-    per_launch_s = ms * 1e-3 / max(launches, 1)
-    bytes_per_launch = alg[dom] * sites * esz // 8
-    achieved = bytes_per_launch / per_launch_s
-    out = {
-        "metric": "site-updates/sec (ModelG 256^3 FP64 vs FP32/batched) and % HBM roofline, 1 GPU",
-        "value": value, "unit": "site-updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": el * 1e3 / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32-storage/f64-arith" if prec == L.FP32 else "f64",
-        "data": "synthetic (phi=0, Gaussian charges per replica)",
-        "config": {"workload": "batched ModelG: 96 replicas (48x32^3, 32x48^3, 16x64^3), "
-                               "one HIP graph per step",
-                   "sites": sites, "dt": DT, "parallelism": "1 GPU",
-                   "ideal_tiles": {str(k): list(v[:2]) for k, v in tuned.items()}},
-        "one_at_a_time": {"value": seq_value, "ms_per_step": seq_el * 1e3 / args.steps},
+    seq_value = sites * steps / seq_el
+    alg, kern, _ = tables(fp32, sites * 10 * esz)
+    return {
+        "value": value, "ms_per_step": el * 1e3 / steps, "steps": steps, "warmup": warmup,
+        "dtype": "f32-storage/f64-arith" if fp32 else "f64",
+        "workload": "batched ModelG: 96 replicas (48x32^3, 32x48^3, 16x64^3), one HIP graph "
+                    "per step",
+        "sites": sites, "ideal_tiles": {str(k): list(v[:2]) for k, v in tuned.items()},
+        "one_at_a_time": {"value": seq_value, "ms_per_step": seq_el * 1e3 / steps},
         "batched_over_one_at_a_time": value / seq_value,
         "step_roofline_frac": value * ALG_BYTES_PER_SITE_UPDATE * esz / 8 / HBM_PEAK,
-        "roofline": {"bound": "hbm", "kernel": kern[dom],
-                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None},
-        "kernels": {k: {"kernel": kern[k], "launches": v[1],
-                        "avg_us": round(v[0] * 1e3 / max(v[1], 1), 2)}
-                    for k, v in tm.items() if k in alg and v[1]},
+        "roofline": roofline_of(tm, alg, kern, sites, esz, "fp32" if fp32 else "fp64"),
+        "kernels": kernel_table(tm, alg, kern, sites, esz),
     }
-    print(json.dumps(out), flush=True)
 
 
-def precision_drift(args, steps=500, every=50):
-    """FP32 fields against FP64 on the same lattice, seed and noise streams:
-    500 steps, a measurement every 50.  Per measurement: <|phi|> = |phi_bar|,
-    |phi_bar|^2 and the k = 1 wall correlator G1 = N^3/3 sum_d sum_a
-    |phi~_a(k=1)|^2 (phi~ from the device r2c of the z/y/x walls, k in units
-    of 2 pi / N).  The susceptibility is N^3 (<|phi_bar|^2> - <|phi_bar|>^2)
-    over the measurement series."""
-    import numpy as np
+def bench_batched(args):
     import langevina_amd as L
-    N = args.size
-    ctxs = {}
-    for name, prec in (("fp64", L.FP64), ("fp32", L.FP32)):
-        c = L.Context(L.model_params(N, seed=42, **PARAMS), precision=prec)
-        c.init_gaussian_charges()
-        c.use_graph(True)
-        ctxs[name] = c
-    series = {k: {"absphi": [], "phi2": [], "G1": []} for k in ctxs}
-    for _ in range(steps // every):
-        for name, c in ctxs.items():
-            c.run(DT, every)
-            m = c.measure_full()
-            pb = m["oavg"][:4]
-            wk = m["wall_k"][:, :4, 1]
-            series[name]["absphi"].append(float(np.sqrt(np.sum(pb * pb))))
-            series[name]["phi2"].append(float(np.sum(pb * pb)))
-            series[name]["G1"].append(float(N ** 3 / 3.0 * np.sum(np.abs(wk) ** 2)))
-    for c in ctxs.values():
-        c.close()
-    out = {"drift": "FP32 fields vs FP64", "N": N, "steps": steps, "measure_every": every,
-           "dt": DT, "params": PARAMS, "seed": 42}
-    for obs in ("absphi", "G1"):
-        a = np.array(series["fp64"][obs])
-        b = np.array(series["fp32"][obs])
-        rel = np.abs(b - a) / np.abs(a)
-        out[obs] = {"fp64": a.tolist(), "fp32": b.tolist(), "max_rel_dev": float(rel.max()),
-                    "final_rel_dev": float(rel[-1])}
-    chi = {k: float(N ** 3 * (np.mean(v["phi2"]) - np.mean(v["absphi"]) ** 2))
-           for k, v in series.items()}
-    out["susceptibility"] = {"fp64": chi["fp64"], "fp32": chi["fp32"],
-                             "rel_dev": abs(chi["fp32"] - chi["fp64"]) / abs(chi["fp64"])}
+    b = run_batched(L, args.precision == "fp32", args.steps, args.warmup,
+                    tune=not args.no_autotune)
+    out = {"metric": METRIC, "value": b.pop("value"), "unit": "site-updates/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": b.pop("ms_per_step"),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": b.pop("dtype"), "data": "synthetic (phi=0, Gaussian charges per replica)",
+           "config": {"workload": b.pop("workload"), "dt": DT, "parallelism": "1 GPU"}}
+    out.update(b)
     print(json.dumps(out), flush=True)
+
+
+def precision_drift(L, N=256, nsteps=500, every=50, seeds=(42, 43, 44, 45)):
+    """FP32 fields against FP64 on the same lattices, seeds and noise streams
+    (the FP64 GPU step equals the oracle's to 1e-12): per seed, nsteps
+    steps from phi = 0 with a measurement every `every`; per measurement
+    <|phi|> = |phi_bar| and the k = 1 wall correlator G1 = V/3 sum_d sum_a
+    |W_a(k=1)|^2 (W the device r2c / N of the z/y/x walls).  Per seed the
+    series means of <|phi|> and G1 and the susceptibility V (<|phi_bar|^2> -
+    <|phi_bar|>^2) over the series; reported as the relative FP32 - FP64
+    difference, mean and standard error over the seeds, beside the
+    seed-to-seed relative spread of the FP64 values (the statistical error
+    of the observable itself)."""
+    import numpy as np
+    per = {k: {"fp64": [], "fp32": []} for k in ("absphi", "G1", "chi")}
+    max_dev = {"absphi": 0.0, "G1": 0.0}
+    V = N ** 3
+    for seed in seeds:
+        ctxs = {}
+        for name, prec in (("fp64", L.FP64), ("fp32", L.FP32)):
+            c = L.Context(L.model_params(N, seed=seed, **PARAMS), precision=prec)
+            c.init_gaussian_charges()
+            c.use_graph(True)
+            ctxs[name] = c
+        series = {k: {"absphi": [], "phi2": [], "G1": []} for k in ctxs}
+        for _ in range(nsteps // every):
+            for name, c in ctxs.items():
+                c.run(DT, every)
+                m = c.measure_full()
+                pb = m["oavg"][:4]
+                wk = m["wall_k"][:, :4, 1]
+                series[name]["absphi"].append(float(np.sqrt(np.sum(pb * pb))))
+                series[name]["phi2"].append(float(np.sum(pb * pb)))
+                series[name]["G1"].append(float(V / 3.0 * np.sum(np.abs(wk) ** 2)))
+        for c in ctxs.values():
+            c.close()
+        for name, ser in series.items():
+            per["absphi"][name].append(float(np.mean(ser["absphi"])))
+            per["G1"][name].append(float(np.mean(ser["G1"])))
+            per["chi"][name].append(float(V * (np.mean(ser["phi2"]) -
+                                               np.mean(ser["absphi"]) ** 2)))
+        for obs in max_dev:
+            a, b = np.array(series["fp64"][obs]), np.array(series["fp32"][obs])
+            max_dev[obs] = max(max_dev[obs], float((np.abs(b - a) / np.abs(a)).max()))
+        progress(f"drift seed {seed}")
+    R = len(seeds)
+    out = {"what": "FP32 fields vs FP64, paired seeds", "N": N, "steps": nsteps,
+           "measure_every": every, "seeds": list(seeds), "dt": DT}
+    for obs, v in per.items():
+        a, b = np.array(v["fp64"]), np.array(v["fp32"])
+        rel = (b - a) / a
+        out[obs] = {"fp64": a.tolist(), "fp32": b.tolist(),
+                    "rel_dev_mean": float(rel.mean()),
+                    "rel_dev_sem": float(rel.std(ddof=1) / np.sqrt(R)) if R > 1 else None,
+                    "fp64_rel_spread": float(a.std(ddof=1) / abs(a.mean())) if R > 1 else None}
+        if obs in max_dev:
+            out[obs]["max_rel_dev_per_measurement"] = max_dev[obs]
+    return out
 
 
 def main():
@@ -317,10 +453,13 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
+                    help="the precision of the headline lattice (fp32: only that form)")
     ap.add_argument("--graph", action="store_true", help="replay a HIP graph per step")
     ap.add_argument("--batched", action="store_true",
-                    help="96 mixed-size replicas in one context (use with --precision fp32)")
+                    help="only the 96 mixed-size replicas in one context")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the 256^3 FP64 lattice (no FP32/batched forms, no drift)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--ideal-tile", type=int, nargs=2, metavar=("CAP", "TZ"),
                     help="fix the ideal-kernel tile instead of autotuning it")
@@ -330,7 +469,7 @@ def main():
     ap.add_argument("--em", action="store_true",
                     help="Model A Euler-Maruyama forward/reverse at 128^3 x 500 steps")
     ap.add_argument("--drift", action="store_true",
-                    help="report FP32-field observables against FP64 (500 steps)")
+                    help="only the FP32-vs-FP64 observable drift (500 steps, paired seeds)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the lattice and every replica "
@@ -339,8 +478,9 @@ def main():
     if args.calibrate_copy:
         calibrate_copy()
         return
+    import langevina_amd as L
     if args.drift:
-        precision_drift(args)
+        print(json.dumps(precision_drift(L, N=args.size)), flush=True)
         return
     if args.em:
         bench_em(args)
@@ -349,94 +489,41 @@ def main():
         bench_batched(args)
         return
 
-    import numpy as np
-    import langevina_amd as L
-
     N = args.size
-    prec = L.FP64 if args.precision == "fp64" else L.FP32
-    esz = 8 if prec == L.FP64 else 4
-    ctx = L.Context(L.model_params(N, seed=42, **PARAMS), precision=prec)
-    # synthetic start: phi = 0 (ModelA::initialize), Gaussian charges with the
-    # zero mode removed (ModelA::initialize_gaussian_charges)
-    ctx.init_gaussian_charges()
-    tuned = {}
-    if args.ideal_tile:
-        from langevina_amd.autotune import set_ideal_tile
-        set_ideal_tile(ctx, 0, *args.ideal_tile)
-        tuned = {N: tuple(args.ideal_tile)}
-    elif not args.no_autotune:
-        # ideal-kernel tile for this lattice size; the tuning steps are warmup
-        from langevina_amd.autotune import autotune_ideal
-        tuned = autotune_ideal(ctx, DT)
-    if args.graph:
-        ctx.use_graph(True)
-    ctx.run(DT, args.warmup)
-    ctx.synchronize()
-
-    ctx.timing_enable(True)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    ctx.run(DT, args.steps)
-    ctx.synchronize()
-    el = time.perf_counter() - t0
-    tm = ctx.timing()
-    ctx.timing_enable(False)
-
-    # the measurement pass (Measurer::measure), reported separately
-    ctx.measure_full()
-    tm0 = time.perf_counter()
-    for _ in range(5):
-        ctx.measure_full()
-    meas_s = (time.perf_counter() - tm0) / 5
-
-    sites = N ** 3
-    value = sites * args.steps / el
-    ms_per_step = el * 1e3 / args.steps
-    # dominant kernel class: the one with the largest total time
-    alg, kern, sched = tables(prec == L.FP32, N ** 3 * 10 * esz)
-    dom = max(("ideal", "phi", "charge"), key=lambda k: tm[k][0])
-    ms, launches = tm[dom]
-    per_launch_s = ms * 1e-3 / max(launches, 1)
-    bytes_per_launch = alg[dom] * sites * esz // 8
-    achieved = bytes_per_launch / per_launch_s
-    kernels = {k: {"kernel": kern[k], "ms_total": round(v[0], 3), "launches": v[1],
-                   "avg_us": round(v[0] * 1e3 / max(v[1], 1), 2),
-                   "alg_GBps": round(alg[k] * sites * esz / 8 /
-                                     (v[0] * 1e-3 / max(v[1], 1)) / 1e9, 1) if v[1] else None}
-               for k, v in tm.items() if k in alg}
+    fp32 = args.precision == "fp32"
+    h = run_single(L, N, fp32, args.steps, args.warmup, graph=args.graph,
+                   tune=not args.no_autotune, ideal_tile=args.ideal_tile, measure=True)
+    progress(f"{h['workload']}: {h['value']:.3e} site-updates/s")
     out = {
-        "metric": "site-updates/sec (ModelG 256^3 FP64 vs FP32/batched) and % HBM roofline, 1 GPU",
-        "value": value,
-        "unit": "site-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if prec == L.FP64 else "f32-storage/f64-arith",
+        "metric": METRIC, "value": h["value"], "unit": "site-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": h["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": h["dtype"],
         "data": "synthetic (phi=0, Gaussian charges, zero mode removed; seed 42)",
-        "config": {"workload": f"ModelG {N}^3 PV2HBSplit23 "
-                               f"{'FP64' if prec == L.FP64 else 'FP32 fields'}",
-                   "N": N, "dt": DT, "params": PARAMS, "parallelism": "1 GPU",
-                   "graph": bool(args.graph),
-                   "ideal_tile": list(tuned[N][:2]) if tuned else "default (32, 16)"},
-        "step_roofline_frac": value * ALG_BYTES_PER_SITE_UPDATE * esz / 8 / HBM_PEAK,
-        "step_schedule_bytes_frac": value * sched * esz / 8 / HBM_PEAK,
-        "roofline": {"bound": "hbm", "kernel": kern[dom],
-                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic(kern[dom], args.precision, N),
-                     "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": per_launch_s * 1e6},
-        "kernels": kernels,
-        "measurement": {"ms": meas_s * 1e3, "alg_GBps": N ** 3 * 10 * esz / meas_s / 1e9,
-                        "what": "lgv_measure_full: walls, phase walls, averages, device r2c "
-                                "of every row, rotate_to_vev; host copies included"},
+        "config": {"workload": h["workload"], "N": N, "dt": DT, "params": PARAMS,
+                   "parallelism": "1 GPU", "graph": bool(args.graph),
+                   "ideal_tile": h["ideal_tile"]},
+        "step_roofline_frac": h["step_roofline_frac"],
+        "step_schedule_bytes_frac": h["step_schedule_bytes_frac"],
+        "roofline": h["roofline"], "kernels": h["kernels"], "measurement": h["measurement"],
     }
+    if not fp32 and not args.headline_only:
+        # the north star's comparison: FP32 fields and the batched ensemble
+        # against this FP64 form, and what FP32 storage does to the observables
+        f = run_single(L, N, True, args.steps, args.warmup, graph=args.graph,
+                       tune=not args.no_autotune)
+        progress(f"{f['workload']}: {f['value']:.3e} site-updates/s")
+        b = run_batched(L, True, args.steps, args.warmup, tune=not args.no_autotune)
+        progress(f"batched FP32: {b['value']:.3e} site-updates/s")
+        for form in (f, b):
+            form["over_fp64"] = form["value"] / h["value"]
+        out["forms"] = {"fp32_fields": f, "batched_fp32": b}
+        out["fp32_drift"] = precision_drift(L, N=N)
+        progress("FP32 drift")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
+        progress("CPU baseline")
     print(json.dumps(out), flush=True)
-    ctx.close()
 
 
 if __name__ == "__main__":

@@ -71,10 +71,19 @@ LGV_HD uint64_t xoro_next(Xoro &g) {
 
 // generate_canonical<double,53> over a 64-bit engine (libstdc++ 11
 // bits/random.tcc:3348-3379): double(x) / 2^64, clamped below one.
-// Multiplying by 2^-64 is the same exact scaling as dividing by 2^64.
+// Multiplying by 2^-64 is the same exact scaling as dividing by 2^64.  On
+// the device double(x) 2^-64 is formed as hi 2^-32 + lo 2^-64 (hi, lo the
+// 32-bit halves) with one rounding -- an FMA whose product and addend are
+// exact -- which is the same number, since scaling by a power of two
+// commutes with rounding; the clamp is a min.
 LGV_HD double canonical(uint64_t x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  const double hi = (double)(uint32_t)(x >> 32) * 0x1p-32;
+  return fmin(fma((double)(uint32_t)x, 0x1p-64, hi), 0x1.fffffffffffffp-1);
+#else
   double r = (double)x * 0x1p-64;
   return r >= 1.0 ? 0x1.fffffffffffffp-1 : r;
+#endif
 }
 
 // uniform_real_distribution(-sqrt(3), sqrt(3)): u*(b-a)+a, b-a = 2 sqrt(3)

@@ -123,17 +123,24 @@ __device__ void block_monitor_add(MonSlot *slot, uint64_t cnt, double dSd, doubl
 }
 
 // ------------------------------------------------------------------------
-// The Metropolis test rr < exp(x), x = -dS <= 0, with the decision of the
-// double exp: a single-precision exp (relative error below 1e-5 for
-// x > -80: one-ulp v_exp_f32 on an argument rounded to float) settles it
-// whenever rr lies outside +-1e-3 of it, the double exp decides the rest.
-__device__ __forceinline__ bool below_exp(double rr, double x) {
-  if (x > -80.) {
-    const double e = (double)__expf((float)x);
-    if (rr < e * (1. - 1e-3)) return true;
-    if (rr >= e * (1. + 1e-3)) return false;
+// The Metropolis test canonical(u) < exp(x) for the draw u and x = -dS <= 0,
+// decided as the double test decides it.  Fast path (x > -11, exp(x) >
+// 1.6e-5): the draw's top 32 bits in single precision against single-
+// precision bounds e (1 -+ 1e-3) 2^32, e = __expf(x).  With r = 2^-32
+// float(u >> 32), |canonical(u) - r| <= 2^-32 + 2^-24 r, and e is within
+// 2e-6 of exp(x) relative (x and x log2(e) rounded to float, one-ulp
+// v_exp_f32), so a draw below the lower bound lies below exp(x) and one at
+// or above the upper bound does not: the margin, about 1e-3 exp(x), exceeds
+// the 2.4e-10 of the draw's rounding whenever exp(x) > 2.4e-7.  The double
+// draw and the double exp decide the rest (about 2e-3 of the tests).
+__device__ __forceinline__ bool accept_draw(uint64_t u, double x) {
+  if (x > -11.) {
+    const float e = __expf((float)x);
+    const float r = (float)(uint32_t)(u >> 32);
+    if (r < e * (0.999f * 0x1p32f)) return true;
+    if (r >= e * (1.001f * 0x1p32f)) return false;
   }
-  return rr < exp(x);
+  return canonical(u) < exp(x);
 }
 
 // The Metropolis decision of one phi site (Stepper.cxx:330-370) in the
@@ -163,8 +170,7 @@ __device__ __forceinline__ void hb_decide(const double phi_o[4], const double he
     accept = true;
     if (count) { cnt += mon_pack(1, 0, 0); dSd += dS; }
   } else {
-    const double rr = uniform01(g);
-    accept = below_exp(rr, -dS);
+    accept = accept_draw(xoro_next(g), -dS);
     if (count) {
       if (accept) { cnt += mon_pack(0, 1, 0); dSu += dS; } else { cnt += mon_pack(0, 0, 1); }
     }
@@ -391,8 +397,7 @@ __device__ __forceinline__ void charge_pair(double nA, double nB, const ChargeK 
   if (dS < 0) {
     if (count) { cnt += mon_pack(1, 0, 0); dSd += dS; }
   } else {
-    const double rr = uniform01(g);
-    if (below_exp(rr, -dS)) {
+    if (accept_draw(xoro_next(g), -dS)) {
       if (count) { cnt += mon_pack(0, 1, 0); dSu += dS; }
     } else {
       if (count) cnt += mon_pack(0, 0, 1);
