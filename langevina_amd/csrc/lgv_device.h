@@ -58,14 +58,28 @@ LGV_HD Xoro xoro_seeded(uint64_t seed) {
   return g;
 }
 
-// xoroshiro128plus::operator(), ModelG/xoroshiro128plus.h:97-107
+// 64-bit rotation right by r < 32 as two 32-bit funnel shifts
+// (v_alignbit_b32), one instruction per half.
+__device__ __forceinline__ uint64_t rotr64_lt32(uint64_t x, int r) {
+  const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+  return ((uint64_t)__builtin_amdgcn_alignbit(lo, hi, r) << 32) |
+         __builtin_amdgcn_alignbit(hi, lo, r);
+}
+
+// xoroshiro128plus::operator(), ModelG/xoroshiro128plus.h:97-107; on the
+// device the rotations by 55 and 36 are taken as rotations right by 9 and 28.
 LGV_HD uint64_t xoro_next(Xoro &g) {
   const uint64_t s0 = g.s0;
   uint64_t s1 = g.s1;
   const uint64_t result = s0 + s1;
   s1 ^= s0;
+#ifdef __HIP_DEVICE_COMPILE__
+  g.s0 = rotr64_lt32(s0, 9) ^ s1 ^ (s1 << 14);
+  g.s1 = rotr64_lt32(s1, 28);
+#else
   g.s0 = rotl64(s0, 55) ^ s1 ^ (s1 << 14);
   g.s1 = rotl64(s1, 36);
+#endif
   return result;
 }
 
@@ -78,7 +92,11 @@ LGV_HD uint64_t xoro_next(Xoro &g) {
 // commutes with rounding; the clamp is a min.
 LGV_HD double canonical(uint64_t x) {
 #ifdef __HIP_DEVICE_COMPILE__
-  const double hi = (double)(uint32_t)(x >> 32) * 0x1p-32;
+  // the empty asm keeps the compiler from re-forming a 64-bit conversion of
+  // x >> 32 (which it lowers with an extra add)
+  uint32_t h = (uint32_t)(x >> 32);
+  __asm__("" : "+v"(h));
+  const double hi = (double)h * 0x1p-32;
   return fmin(fma((double)(uint32_t)x, 0x1p-64, hi), 0x1.fffffffffffffp-1);
 #else
   double r = (double)x * 0x1p-64;

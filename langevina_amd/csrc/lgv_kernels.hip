@@ -72,6 +72,24 @@ __device__ __forceinline__ GPlane<T, W> gplane(const ReplicaParams &p, int cur, 
   char *base = reinterpret_cast<char *>(buf ? p.fields[1] : p.fields[0]);
   return {(gbyte *)(base + (int64_t)(c * 2 + par) * p.half * (int64_t)sizeof(T))};
 }
+// The two field buffers of a replica read once into registers: a kernel
+// that stores between its loads then forms every plane base from these
+// values, where gplane(p, ...) would reload p.fields after each store (the
+// compiler cannot prove that a field store misses the parameter block).
+struct FieldBufs {
+  char *f0, *f1;
+  int64_t half_bytes;  // one half-plane
+};
+template <typename T>
+__device__ __forceinline__ FieldBufs field_bufs(const ReplicaParams &p) {
+  return {reinterpret_cast<char *>(p.fields[0]), reinterpret_cast<char *>(p.fields[1]),
+          p.half * (int64_t)sizeof(T)};
+}
+template <typename T, bool W>
+__device__ __forceinline__ GPlane<T, W> gplane(const FieldBufs &fb, int cur, int c, int par) {
+  const int buf = c < 4 ? (cur & 1) : (cur >> 1);
+  return {(gbyte *)((buf ? fb.f1 : fb.f0) + (int64_t)(c * 2 + par) * fb.half_bytes)};
+}
 
 // Block index of this workgroup within the launch plan.  (A renumbering that
 // gives each XCD a contiguous eighth of the plan was measured on MI355X and
@@ -438,6 +456,7 @@ __device__ __forceinline__ void charge_line(const ReplicaParams &p, int cur, int
     lin[k] = (zz * N + yy) * N + xx;
   }
   const ChargeK ck = charge_k(p.rms, p.chi);
+  const FieldBufs fb = field_bufs<T>(p);
   Xoro gL = xoro_seeded(site_key(k1, lin[0]));
   Xoro gB = xoro_seeded(site_key(k1, lin[2]));
   Xoro gA = xoro_seeded(site_key(k2, lin[1]));
@@ -447,7 +466,7 @@ __device__ __forceinline__ void charge_line(const ReplicaParams &p, int cur, int
   double w[6][4];
 #pragma unroll
   for (int L = 4; L < 10; L++) {
-    const GPlane<T, W> f1 = gplane<T, W>(p, cur, L, e1), f2 = gplane<T, W>(p, cur, L, e2);
+    const GPlane<T, W> f1 = gplane<T, W>(fb, cur, L, e1), f2 = gplane<T, W>(fb, cur, L, e2);
     w[L - 4][0] = f1.ld(ii[0]);
     w[L - 4][1] = f2.ld(ii[1]);
     w[L - 4][2] = f1.ld(ii[2]);
@@ -467,8 +486,8 @@ __device__ __forceinline__ void charge_line(const ReplicaParams &p, int cur, int
     b1 = (double)(T)b1;
     double a2, b2;
     charge_pair(a1, b1, ck, gA, a2, b2, cnt, dSd, dSu, true);
-    gplane<T, W>(p, nxt, L, e2).st(ii[1], a2);
-    gplane<T, W>(p, nxt, L, e1).st(ii[2], b2);
+    gplane<T, W>(fb, nxt, L, e2).st(ii[1], a2);
+    gplane<T, W>(fb, nxt, L, e1).st(ii[2], b2);
   }
 }
 
@@ -489,6 +508,7 @@ __device__ __forceinline__ void charge_row_wave(const ReplicaParams &p, int cur,
   const uint32_t N = p.N, NH = p.NH;
   const uint32_t row = z * N + y;
   const ChargeK ck = charge_k(p.rms, p.chi);
+  const FieldBufs fb = field_bufs<T>(p);
   // a1: updated A'_h; r: updated L_{h+1} from lane + 1; l0: lane 0's updated L
   double a1[H][6], r[H][6], l0[H][6];
 #pragma unroll
@@ -500,8 +520,8 @@ __device__ __forceinline__ void charge_row_wave(const ReplicaParams &p, int cur,
     double wl[6], wa[6];
 #pragma unroll
     for (int L = 4; L < 10; L++) {
-      wl[L - 4] = gplane<T, W>(p, cur, L, e1).ld(iL);
-      wa[L - 4] = gplane<T, W>(p, cur, L, e2).ld(iA);
+      wl[L - 4] = gplane<T, W>(fb, cur, L, e1).ld(iL);
+      wa[L - 4] = gplane<T, W>(fb, cur, L, e2).ld(iA);
     }
     Xoro gL = xoro_seeded(site_key(k1, row * N + xl));
 #pragma unroll
@@ -532,8 +552,8 @@ __device__ __forceinline__ void charge_row_wave(const ReplicaParams &p, int cur,
       const double nb = lane == 63 ? l0[(j + 1) % H][c] : r[j][c];
       double a2, b2;
       charge_pair(a1[j][c], nb, ck, gA, a2, b2, cnt, dSd, dSu, true);
-      gplane<T, W>(p, nxt, 4 + c, e2).st(row * NH + h, a2);
-      gplane<T, W>(p, nxt, 4 + c, e1).st(row * NH + (xb >> 1), b2);
+      gplane<T, W>(fb, nxt, 4 + c, e2).st(row * NH + h, a2);
+      gplane<T, W>(fb, nxt, 4 + c, e1).st(row * NH + (xb >> 1), b2);
     }
   }
 }
@@ -557,6 +577,7 @@ __device__ __forceinline__ void charge_column(const ReplicaParams &p, int cur, i
                                               double &dSd, double &dSu) {
   const int N = p.N, NH = p.NH;
   const ChargeK ck = charge_k(p.rms, p.chi);
+  const FieldBufs fb = field_bufs<T>(p);
   // index / linear site of position m along the column
   auto at = [&](int m, uint32_t &ii, uint32_t &lin) __attribute__((always_inline)) {
     const uint32_t row = dir == 2 ? (uint32_t)m * N + q : q * N + (uint32_t)m;
@@ -589,8 +610,8 @@ __device__ __forceinline__ void charge_column(const ReplicaParams &p, int cur, i
     double w1[6], w2[6];
 #pragma unroll
     for (int L = 4; L < 10; L++) {
-      w1[L - 4] = gplane<T, W>(p, cur, L, e1).ld(i1);
-      w2[L - 4] = gplane<T, W>(p, cur, L, e2).ld(i2);
+      w1[L - 4] = gplane<T, W>(fb, cur, L, e1).ld(i1);
+      w2[L - 4] = gplane<T, W>(fb, cur, L, e2).ld(i2);
     }
 #pragma unroll
     for (int L = 4; L < 10; L++) {
@@ -603,8 +624,8 @@ __device__ __forceinline__ void charge_column(const ReplicaParams &p, int cur, i
       if (k > 0) {
         double o0, o1;
         charge_pair(up[L - 4], a, ck, g2, o0, o1, cnt, dSd, dSu, true);
-        gplane<T, W>(p, nxt, L, e2).st(i0, o0);
-        gplane<T, W>(p, nxt, L, e1).st(i1, o1);
+        gplane<T, W>(fb, nxt, L, e2).st(i0, o0);
+        gplane<T, W>(fb, nxt, L, e1).st(i1, o1);
       }
       up[L - 4] = b;
     }
