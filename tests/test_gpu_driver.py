@@ -41,6 +41,10 @@ def run_driver(tmp, params, name="in.json"):
 
 
 def oracle_run(orc, par, nsteps_run, lat=None, thermalize=True):
+    """run_event (SuperPions.cxx:78-161) on the oracle: the clock reset,
+    thermalize_event (with the quench of SuperPions.cxx:31-75 when
+    quench_mode is set: mass0 = quench_mode_mass0 and dmassdt = 0 while
+    thermalizing, then restored), then measure-before-step."""
     N = par["NX"]
     if lat is None:
         lat = orc.Lattice(N, mass0=par["mass0"], dmassdt=par["dmassdt"], lam=par["lambda"],
@@ -50,7 +54,12 @@ def oracle_run(orc, par, nsteps_run, lat=None, thermalize=True):
     t = float(par["initialtime"])
     lat.t = t
     if thermalize:
+        quench = par.get("quench_mode", False)
+        if quench:
+            lat.c.mass0, lat.c.dmassdt = par["quench_mode_mass0"], 0.0
         lat.thermalize(dt, int(par["thermalization_time"] / dt))
+        if quench:
+            lat.c.mass0, lat.c.dmassdt = par["mass0"], par["dmassdt"]
     rows = []
     for steps in range(nsteps_run):
         if steps % par["saveFrequency"] == 0:
@@ -132,7 +141,7 @@ def test_driver_restart_continues_the_stream(tmp_path, orc):
     np.testing.assert_allclose(phi[5:], np.array([r[2] for r in rows2]), rtol=1e-10, atol=1e-13)
 
 
-def test_driver_event_and_quench_modes(tmp_path):
+def test_driver_event_and_quench_modes(tmp_path, orc):
     log("events")
     tmp = str(tmp_path)
     par = dict(PAR, eventmode=True, nevents=2, quench_mode=True, quench_mode_mass0=-4.0)
@@ -142,3 +151,22 @@ def test_driver_event_and_quench_modes(tmp_path):
     assert not os.path.exists(os.path.join(tmp, "run_save.h5"))  # no save in event mode
     assert out.count("Settinng up a quench initial condition") == 2
     assert h5_read(os.path.join(tmp, "run_0001.h5"), "/phi").shape == (5, 11)
+    # the oracle replays both events: each thermalizes the previous event's
+    # final fields at the quench mass (fresh charges), then runs at the
+    # nominal m^2(t); the measured rows and k-space walls must agree
+    lat = None
+    for ev in range(2):
+        lat, rows = oracle_run(orc, par, 10, lat=lat)
+        h5 = os.path.join(tmp, f"run_{ev:04d}.h5")
+        phi = h5_read(h5, "/phi")
+        timeout = h5_read(h5, "/timeout")
+        wk = h5_read(h5, "/wallz_k")
+        assert phi.shape == (len(rows), 11)
+        for i, (t, m, oa, w, wp) in enumerate(rows):
+            assert timeout[i, 0] == t and abs(timeout[i, 1] - m) <= 1e-15, (ev, i)
+            np.testing.assert_allclose(phi[i], oa, rtol=1e-10, atol=1e-13, err_msg=f"event {ev}")
+            np.testing.assert_allclose(wk[i], orc.dft_rows(w[2]), rtol=1e-10, atol=1e-13)
+    # the quench changed the thermalized state: the same run without it differs
+    lat_nq, rows_nq = oracle_run(orc, dict(par, quench_mode=False), 10)
+    phi0 = h5_read(os.path.join(tmp, "run_0000.h5"), "/phi")
+    assert np.abs(phi0[0] - rows_nq[0][2]).max() > 1e-6

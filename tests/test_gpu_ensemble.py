@@ -12,9 +12,12 @@ cold start; the GPU replicas measure once per 100 sweeps.
   (tests/golden/hb_ensemble.json, written by make_hb_ensemble.py): <phi_0>,
   <phi^2> and <|phi_bar|> agree within 2 combined standard errors, each
   error from the spread of independent chains.
-* The stored reference value <phi_0> = 0.29771577381 +- 0.00033803
-  (ModelG/hcrit1_N032_m-04813_h010000_l10000_averages_ave.out:1) is printed
-  with its distance in standard errors and held to 1 % relative.  The oracle
+* The stored reference values (ModelG/hcrit1_N032_m-04813_h010000_l10000_
+  averages_ave.out:1): <phi_0> = 0.29771577381 +- 0.00033803, <phi^2> and
+  <|phi_bar|> are printed with their distance in standard errors and held
+  to 1 % relative; phi_1..3, zero by the O(3) symmetry left by H along
+  phi_0, are held within 4 standard errors of the stored values (combined)
+  and of zero.  The oracle
   in reference order (14 chains) and the GPU both sit about 3 of its
   standard errors below it: the run that produced it is not in the
   reference tree (its tag reads lambda = 1.0000, a normalisation the current
@@ -29,6 +32,15 @@ import numpy as np
 import pytest
 
 REF_MEAN, REF_ERR = 0.29771577381, 3.3803070885e-04
+# The other columns of that file (mean, error, integrated autocorrelation
+# time / 100 per observable): phi_1..3, then 0.99771 +- 0.00005, which is
+# the volume average of phi^2, and 0.30236 +- 0.00030, which is |phi_bar|
+# (the sixth, 1.2372, matches no quantity the measurer stores).
+REF_COLS = {"phi_1": (-6.3950531535e-04, 1.2292241007e-03),
+            "phi_2": (-3.5453750383e-04, 1.2086034345e-03),
+            "phi_3": (-8.6118122099e-04, 1.1829254410e-03),
+            "phi2": (9.9771450882e-01, 4.8322472767e-05),
+            "absphi": (3.0235983190e-01, 2.9856553209e-04)}
 
 
 def autocorrelation(data, tmax):
@@ -77,7 +89,7 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "hb_
 
 
 def replica_ensemble(R=48, therm=4000, nmeas=2000, every=100):
-    """Per-replica autocorrelation-corrected means of phi_0, phi^2, |phi_bar|."""
+    """Per-replica series of phi_0, phi^2, |phi_bar|, phi_1, phi_2, phi_3."""
     import langevina_amd as L
     g = json.load(open(GOLDEN))["params"]
     reps = [L.model_params(g["N"], mass0=g["mass0"], lam=g["lam"], H=g["H"], chi=g["chi"],
@@ -85,7 +97,7 @@ def replica_ensemble(R=48, therm=4000, nmeas=2000, every=100):
     ctx = L.Context(reps)
     for _ in range(therm):
         ctx.hb_phi_step(g["dt"])
-    obs = np.zeros((3, R, nmeas))
+    obs = np.zeros((6, R, nmeas))
     for i in range(nmeas):
         for _ in range(every):
             ctx.hb_phi_step(g["dt"])
@@ -94,6 +106,7 @@ def replica_ensemble(R=48, therm=4000, nmeas=2000, every=100):
             obs[0, r, i] = o[0]
             obs[1, r, i] = o[10]
             obs[2, r, i] = math.sqrt(float(np.sum(o[:4] ** 2)))
+            obs[3:6, r, i] = o[1:4]
     ctx.close()
     return obs
 
@@ -117,5 +130,21 @@ def test_heatbath_ensemble_matches_oracle_and_reference(capsys):
                          f"{(mean - REF_MEAN) / ref_sig:+.2f} sigma, "
                          f"{(mean - REF_MEAN) / REF_MEAN:+.3%}")
             assert abs(mean - REF_MEAN) < 0.01 * REF_MEAN, lines[-1]
+    for k, name in ((1, "phi2"), (2, "absphi")):
+        means = np.array([autocorrelated_err(obs[k, r])[0] for r in range(obs.shape[1])])
+        mean, err = means.mean(), means.std(ddof=1) / math.sqrt(len(means))
+        rm, re = REF_COLS[name]
+        lines.append(f"{name} vs stored reference {rm:.6f} +- {re:.6f}: "
+                     f"{(mean - rm) / math.hypot(err, re):+.2f} sigma, {(mean - rm) / rm:+.3%}")
+        assert abs(mean - rm) < 0.01 * rm, lines[-1]
+    for k, name in ((3, "phi_1"), (4, "phi_2"), (5, "phi_3")):
+        means = np.array([autocorrelated_err(obs[k, r])[0] for r in range(obs.shape[1])])
+        mean, err = means.mean(), means.std(ddof=1) / math.sqrt(len(means))
+        rm, re = REF_COLS[name]
+        sig = math.hypot(err, re)
+        lines.append(f"{name}: GPU {mean:+.6f} +- {err:.6f}, "
+                     f"stored {rm:+.6f} +- {re:.6f} ({(mean - rm) / sig:+.2f} sigma)")
+        assert abs(mean - rm) < 4 * sig, lines[-1]
+        assert abs(mean) < 4 * err, lines[-1]
     with capsys.disabled():
         print("\n" + "\n".join(lines), flush=True)
