@@ -27,13 +27,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 # Algorithmic bytes per site per launch, FP64 fields (halve for FP32):
 # ideal substep (k_ideal_fused) read 80 + write 80; phi half-sweep
-# (k_hb_march for FP64 fields, k_hb_phi for FP32) read all phi 32 + write
+# (k_hb_march for large fields, k_hb_phi otherwise) read all phi 32 + write
 # one parity 16; charge direction (k_charge_march, both sub-sweeps of one
 # direction) read 48 + write 48.
 ALG_BYTES = {"ideal": 160, "phi": 48, "charge": 96}
 KERNEL = {"ideal": "k_ideal_fused", "phi": "k_hb_march", "charge": "k_charge_march"}
-ALG_BYTES_FP32 = ALG_BYTES
-KERNEL_FP32 = dict(KERNEL, phi="k_hb_phi")
 # Per site-update with one pass per dependency level of the reference
 # (12 phi half-sweeps of 48 B, 6 charge sub-sweeps of 96 B): 3*160 + 12*48 + 6*96.
 ALG_BYTES_PER_SITE_UPDATE = 1632
@@ -54,20 +52,24 @@ MEASURED_TRAFFIC = {
 }
 
 
-# The library runs the phi z-march for FP64 fields above 1 GiB (one buffer,
-# all replicas) and the one-thread-per-site phi kernel otherwise.
-LARGE_FIELD_BYTES = 2 ** 30
+# The library runs the phi z-march for FP64 fields above 1 GiB and FP32
+# fields above 600 MiB (one buffer, all replicas) and the one-thread-per-site
+# phi kernel otherwise.
+LARGE_FIELD_BYTES = {False: 2 ** 30, True: 600 * 2 ** 20}
 
 
-def tables(fp32, field_bytes=None):
+def phi_march(fp32, field_bytes):
+    return field_bytes > LARGE_FIELD_BYTES[fp32]
+
+
+def tables(fp32, field_bytes):
     """(algorithmic bytes per site per launch, kernel name, schedule bytes per
     site-update) for the precision's kernel schedule; field_bytes (one
     buffer) decides the phi kernel."""
-    if fp32:
-        return ALG_BYTES_FP32, KERNEL_FP32, SCHEDULE_BYTES_PER_SITE_UPDATE_FP32
-    if field_bytes is not None and field_bytes <= LARGE_FIELD_BYTES:
-        return ALG_BYTES, dict(KERNEL, phi="k_hb_phi"), SCHEDULE_BYTES_PER_SITE_UPDATE
-    return ALG_BYTES, KERNEL, SCHEDULE_BYTES_PER_SITE_UPDATE
+    sched = SCHEDULE_BYTES_PER_SITE_UPDATE_FP32 if fp32 else SCHEDULE_BYTES_PER_SITE_UPDATE
+    if phi_march(fp32, field_bytes):
+        return ALG_BYTES, KERNEL, sched
+    return ALG_BYTES, dict(KERNEL, phi="k_hb_phi"), sched
 
 
 def traffic(kernel, precision, N):
@@ -277,10 +279,13 @@ def run_single(L, N, fp32, steps, warmup, graph=False, tune=True, ideal_tile=Non
         from langevina_amd.autotune import set_ideal_tile
         set_ideal_tile(ctx, 0, *ideal_tile)
         tuned = {N: tuple(ideal_tile)}
-    elif tune:
-        # ideal-kernel tile for this lattice size; the tuning steps are warmup
-        from langevina_amd.autotune import autotune_ideal
-        tuned = autotune_ideal(ctx, DT)
+    sweeps = {}
+    if tune:
+        # kernel tiles for this lattice size; the tuning steps are warmup
+        from langevina_amd.autotune import autotune_ideal, autotune_sweeps
+        if not ideal_tile:
+            tuned = autotune_ideal(ctx, DT)
+        sweeps = autotune_sweeps(ctx, DT, march=phi_march(fp32, N ** 3 * 10 * esz))
     if graph:
         ctx.use_graph(True)
     ctx.run(DT, warmup)
@@ -301,6 +306,7 @@ def run_single(L, N, fp32, steps, warmup, graph=False, tune=True, ideal_tile=Non
         "dtype": "f32-storage/f64-arith" if fp32 else "f64",
         "workload": f"ModelG {N}^3 PV2HBSplit23 {'FP32 fields' if fp32 else 'FP64'}",
         "ideal_tile": list(tuned[N][:2]) if tuned else "default (32, 16)",
+        "sweep_tiles": list(sweeps[N]) if sweeps else "default",
         "step_roofline_frac": value * ALG_BYTES_PER_SITE_UPDATE * esz / 8 / HBM_PEAK,
         "step_schedule_bytes_frac": value * sched * esz / 8 / HBM_PEAK,
         "roofline": roofline_of(tm, alg, kern, sites, esz, "fp32" if fp32 else "fp64", N),
@@ -323,7 +329,8 @@ def run_single(L, N, fp32, steps, warmup, graph=False, tune=True, ideal_tile=Non
 def run_batched(L, fp32, steps, warmup, tune=True):
     """The 96-replica ensemble in one context, one HIP graph per step, against
     the same replicas one context (and one graph) at a time."""
-    from langevina_amd.autotune import autotune_ideal, set_ideal_tile
+    from langevina_amd.autotune import (autotune_ideal, autotune_sweeps, set_ideal_tile,
+                                        set_sweep_tiles)
     reps = batched_replicas(L)
     sites = sum(p.NX ** 3 for p in reps)
     prec = L.FP32 if fp32 else L.FP64
@@ -332,6 +339,9 @@ def run_batched(L, fp32, steps, warmup, tune=True):
     for r in range(len(reps)):
         ctx.init_gaussian_charges(r)
     tuned = autotune_ideal(ctx, DT) if tune else {}
+    sweeps = {}
+    if tune:
+        sweeps = autotune_sweeps(ctx, DT, march=phi_march(fp32, sites * 10 * esz))
     ctx.timing_enable(True)  # kernel breakdown from a short direct-launch pass
     ctx.run(DT, 5)
     tm = ctx.timing()
@@ -352,6 +362,7 @@ def run_batched(L, fp32, steps, warmup, tune=True):
         if tuned:
             cap, tz = tuned[p.NX][:2]
             set_ideal_tile(c1, 0, cap, tz)
+            set_sweep_tiles(c1, 0, *sweeps[p.NX])
         c1.use_graph(True)
         c1.run(DT, 2)
         c1.synchronize()
@@ -368,6 +379,7 @@ def run_batched(L, fp32, steps, warmup, tune=True):
         "workload": "batched ModelG: 96 replicas (48x32^3, 32x48^3, 16x64^3), one HIP graph "
                     "per step",
         "sites": sites, "ideal_tiles": {str(k): list(v[:2]) for k, v in tuned.items()},
+        "sweep_tiles": {str(k): list(v) for k, v in sweeps.items()},
         "one_at_a_time": {"value": seq_value, "ms_per_step": seq_el * 1e3 / steps},
         "batched_over_one_at_a_time": value / seq_value,
         "step_roofline_frac": value * ALG_BYTES_PER_SITE_UPDATE * esz / 8 / HBM_PEAK,
@@ -502,7 +514,7 @@ def main():
         "data": "synthetic (phi=0, Gaussian charges, zero mode removed; seed 42)",
         "config": {"workload": h["workload"], "N": N, "dt": DT, "params": PARAMS,
                    "parallelism": "1 GPU", "graph": bool(args.graph),
-                   "ideal_tile": h["ideal_tile"]},
+                   "ideal_tile": h["ideal_tile"], "sweep_tiles": h["sweep_tiles"]},
         "step_roofline_frac": h["step_roofline_frac"],
         "step_schedule_bytes_frac": h["step_schedule_bytes_frac"],
         "roofline": h["roofline"], "kernels": h["kernels"], "measurement": h["measurement"],

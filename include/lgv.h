@@ -174,6 +174,14 @@ int lgv_monitor_reset(lgv_ctx *ctx);
  * at most txh_cap half-columns (2..64) in x, 256/txh rows in y, tz planes
  * per z-chunk.  Results do not depend on it; the autotuner picks it. */
 int lgv_set_ideal_tile(lgv_ctx *ctx, int replica, int txh_cap, int tz);
+/* Tiles of the two Metropolis sweep kernels for one replica (replica < 0:
+ * all): the phi z-march (at most hb_cap half-columns, 4..64, x 256/hb_cap
+ * rows, chunks of hb_tz planes) and the charge march (at most chg_cap
+ * half-columns, 4..64, x 256/chg_cap lines, chunks of chg_chunk sites, even).
+ * 0 keeps a default.  Results do not depend on them; the autotuner picks
+ * them.  Monitor counts are kept across the change. */
+int lgv_set_sweep_tiles(lgv_ctx *ctx, int replica, int hb_cap, int hb_tz, int chg_cap,
+                        int chg_chunk);
 
 int lgv_synchronize(lgv_ctx *ctx);
 /* Device address of a replica's field storage (10 component planes, each

@@ -2,6 +2,7 @@
 // composition of PV2HBSplit::step from the kernels, host-side global engine
 // (initial charges), monitors and measurement.
 #include <cfloat>
+#include <array>
 #include <cmath>
 #include <complex>
 #include <limits>
@@ -95,6 +96,9 @@ struct lgv_ctx {
   bool nt_stores = false;  // one buffer's fields exceed kStreamBytes
   bool phi_march = false;  // the phi half-sweep runs the z-march kernel
   MonSlot *mon_phi = nullptr, *mon_charge = nullptr, *mon_ideal = nullptr;
+  // monitor counts folded out of the per-block slots when a retile
+  // reallocated them: [replica][phi, charge]
+  std::vector<std::array<lgv_monitor, 2>> mon_carry;
   EnergySlot *eslots = nullptr;
   double coef_hb = NAN, coef_charge = NAN, coef_ideal = NAN;
   double *d_aos = nullptr;
@@ -163,6 +167,14 @@ int build_plan(Plan &plan, const std::vector<int64_t> &work, bool wide) {
   return LGV_OK;
 }
 
+// Releases a plan's device arrays (the stream must be idle).
+int free_plan(Plan &plan) {
+  if (plan.d_blk_rep) HIPCHECK(hipFree(plan.d_blk_rep));
+  if (plan.d_blk_first) HIPCHECK(hipFree(plan.d_blk_first));
+  plan = Plan{};
+  return LGV_OK;
+}
+
 // Blocks of the fused ideal kernel: one per (x-tile, y-tile, z-chunk) of
 // every replica, with that replica's tile shape.
 int build_ideal_plan(lgv_ctx *c) {
@@ -193,13 +205,15 @@ int build_hb_plan(lgv_ctx *c) {
   for (int r = 0; r < c->nrep; r++) {
     const ReplicaParams &p = c->hp[r];
     int TXH, TY, TZ;
-    hb_tile(p.N, TXH, TY, TZ);
+    hb_tile(p.N, p.hb_cap, p.hb_tz, TXH, TY, TZ);
     if ((TY + 2) * (TXH + 2) > kHbMarchGrid || 2 * TY + 2 * TXH > kBlock)
       return fail(LGV_ERR_ARG, "replica %d: phi tile %dx%d does not fit", r, TXH, TY);
     work.push_back((int64_t)((p.NH + TXH - 1) / TXH) * ((p.N + TY - 1) / TY) *
                    ((p.N + TZ - 1) / TZ) * kBlock);
   }
-  const int rc = build_plan(c->plan_hb, work, c->wide);
+  int rc = free_plan(c->plan_hb);
+  if (rc) return rc;
+  rc = build_plan(c->plan_hb, work, c->wide);
   c->plan_hb.stream = c->nt_stores;
   return rc;
 }
@@ -210,14 +224,23 @@ int build_charge_plan(lgv_ctx *c) {
   for (int r = 0; r < c->nrep; r++) {
     const ReplicaParams &p = c->hp[r];
     int TXH, TQ, C;
-    charge_tile(p.N, TXH, TQ, C);
+    charge_tile(p.N, p.chg_cap, p.chg_chunk, TXH, TQ, C);
+    if (C % 2 != 0) return fail(LGV_ERR_ARG, "replica %d: charge chunk %d is odd", r, C);
     work.push_back((int64_t)((p.NH + TXH - 1) / TXH) * ((p.N + TQ - 1) / TQ) *
                    ((p.N + C - 1) / C) * kBlock);
   }
-  const int rc = build_plan(c->plan_charge, work, c->wide);
-  // the x sub-sweeps with whole rows per wavefront need NH = 64 or 128
+  int rc = free_plan(c->plan_charge);
+  if (rc) return rc;
+  rc = build_plan(c->plan_charge, work, c->wide);
+  // the x sub-sweeps with whole rows per wavefront need NH = 64 or 128 and
+  // a tile of 64 half-columns x 4 lines
   bool xw = true;
-  for (int r = 0; r < c->nrep; r++) xw = xw && (c->hp[r].NH == 64 || c->hp[r].NH == 128);
+  for (int r = 0; r < c->nrep; r++) {
+    const ReplicaParams &p = c->hp[r];
+    int TXH, TQ, C;
+    charge_tile(p.N, p.chg_cap, p.chg_chunk, TXH, TQ, C);
+    xw = xw && (p.NH == 64 || p.NH == 128) && TXH == 64 && TQ == 4;
+  }
   c->plan_charge.xwave = xw;
   return rc;
 }
@@ -443,6 +466,7 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
     }
     p.tile_cap = 32;
     p.tile_tz = 16;
+    p.hb_cap = p.hb_tz = p.chg_cap = p.chg_chunk = 0;
     p.N = N;
     p.NH = N / 2;
     p.half = half;
@@ -471,7 +495,7 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
   c->nt_stores = buffer_bytes > kStreamBytes;
   // test hook: LGV_NT_STORES=0/1 forces the store mode on any size
   if (const char *nt = getenv("LGV_NT_STORES")) c->nt_stores = atoi(nt) != 0;
-  c->phi_march = !c->fp32 && c->nt_stores;
+  c->phi_march = c->fp32 ? buffer_bytes > kMarchBytesFp32 : c->nt_stores;
   // test hook: LGV_PHI_KERNEL=site/march forces the phi half-sweep kernel
   if (const char *pk = getenv("LGV_PHI_KERNEL")) {
     const std::string k(pk);
@@ -496,6 +520,7 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
   HIPCHECK(hipMemset(c->mon_charge, 0, c->plan_charge.nblocks * sizeof(MonSlot)));
   HIPCHECK(hipMalloc(&c->mon_ideal, c->nrep * sizeof(MonSlot)));
   HIPCHECK(hipMemset(c->mon_ideal, 0, c->nrep * sizeof(MonSlot)));
+  c->mon_carry.assign(c->nrep, {lgv_monitor{}, lgv_monitor{}});
   HIPCHECK(hipMalloc(&c->eslots, c->plan_full.nblocks * sizeof(EnergySlot)));
   HIPCHECK(hipMalloc(&c->d_aos, (int64_t)stage_planes(c->maxN) * c->maxN * c->maxN * 10 *
                                      sizeof(double)));
@@ -717,7 +742,7 @@ int lgv_monitor_get(lgv_ctx *c, int r, lgv_monitor out[3]) {
     std::vector<MonSlot> slots(b1 - b0);
     HIPCHECK(hipMemcpy(slots.data(), src[k] + b0, slots.size() * sizeof(MonSlot),
                        hipMemcpyDeviceToHost));
-    lgv_monitor m{};
+    lgv_monitor m = c->mon_carry[r][k];
     for (const MonSlot &s : slots) {
       m.down_counts += s.down_counts;
       m.up_counts += s.up_counts;
@@ -960,6 +985,59 @@ int lgv_set_ideal_tile(lgv_ctx *c, int r, int txh_cap, int tz) {
   return LGV_OK;
 }
 
+int lgv_set_sweep_tiles(lgv_ctx *c, int r, int hb_cap, int hb_tz, int chg_cap,
+                        int chg_chunk) {
+  if (!c) return fail(LGV_ERR_ARG, "null context");
+  if (r >= c->nrep) return fail(LGV_ERR_RANGE, "replica out of range");
+  auto bad = [](int v, int lo, int hi) { return v != 0 && (v < lo || v > hi); };
+  if (bad(hb_cap, 4, 64) || bad(hb_tz, 1, 1024) || bad(chg_cap, 4, 64) ||
+      bad(chg_chunk, 2, 1024) || chg_chunk % 2 != 0)
+    return fail(LGV_ERR_ARG, "sweep tile %d/%d, %d/%d out of range", hb_cap, hb_tz, chg_cap,
+                chg_chunk);
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  // the per-block monitor slots follow the plans: fold them into the
+  // per-replica carry before they are reallocated
+  {
+    lgv_monitor m[3];
+    for (int i = 0; i < c->nrep; i++) {
+      int rc = lgv_monitor_get(c, i, m);
+      if (rc) return rc;
+      c->mon_carry[i] = {m[0], m[1]};
+    }
+  }
+  std::vector<ReplicaParams> old = c->hp;
+  for (int i = 0; i < c->nrep; i++) {
+    if (r >= 0 && i != r) continue;
+    c->hp[i].hb_cap = hb_cap;
+    c->hp[i].hb_tz = hb_tz;
+    c->hp[i].chg_cap = chg_cap;
+    c->hp[i].chg_chunk = chg_chunk;
+  }
+  int rc = build_hb_plan(c);
+  if (!rc) rc = build_charge_plan(c);
+  if (rc) {
+    c->hp = old;
+    build_hb_plan(c);
+    build_charge_plan(c);
+  }
+  HIPCHECK(hipFree(c->mon_phi));
+  HIPCHECK(hipFree(c->mon_charge));
+  c->mon_phi = c->mon_charge = nullptr;
+  HIPCHECK(hipMalloc(&c->mon_phi, hb_plan(c).nblocks * sizeof(MonSlot)));
+  HIPCHECK(hipMemset(c->mon_phi, 0, hb_plan(c).nblocks * sizeof(MonSlot)));
+  HIPCHECK(hipMalloc(&c->mon_charge, c->plan_charge.nblocks * sizeof(MonSlot)));
+  HIPCHECK(hipMemset(c->mon_charge, 0, c->plan_charge.nblocks * sizeof(MonSlot)));
+  HIPCHECK(hipMemcpy(c->dP, c->hp.data(), c->nrep * sizeof(ReplicaParams),
+                     hipMemcpyHostToDevice));
+  HIPCHECK(hipDeviceSynchronize());
+  if (c->graph_exec) {
+    HIPCHECK(hipGraphExecDestroy(c->graph_exec));
+    c->graph_exec = nullptr;
+  }
+  return rc;
+}
+
 int lgv_monitor_reset(lgv_ctx *c) {
   if (!c) return fail(LGV_ERR_ARG, "null context");
   HIPCHECK(hipSetDevice(c->device));
@@ -968,6 +1046,7 @@ int lgv_monitor_reset(lgv_ctx *c) {
   HIPCHECK(hipMemset(c->mon_charge, 0, c->plan_charge.nblocks * sizeof(MonSlot)));
   HIPCHECK(hipMemset(c->mon_ideal, 0, c->nrep * sizeof(MonSlot)));
   HIPCHECK(hipDeviceSynchronize());
+  for (auto &cr : c->mon_carry) cr = {lgv_monitor{}, lgv_monitor{}};
   return LGV_OK;
 }
 

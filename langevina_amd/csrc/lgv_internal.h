@@ -28,7 +28,9 @@ struct ReplicaParams {
   double dt_ideal;    // dt / scounts[1]
   double tau_ideal;   // dt_ideal / 2.0
   uint64_t base_key;
-  int tile_cap, tile_tz;  // ideal-kernel tile (see ideal_tile)
+  int tile_cap, tile_tz;   // ideal-kernel tile (see ideal_tile)
+  int hb_cap, hb_tz;       // phi-march tile (see hb_tile; 0 = default)
+  int chg_cap, chg_chunk;  // charge-march tile (see charge_tile; 0 = default)
 };
 
 // Device-written state of one replica (clock, counters, per-step data).
@@ -60,31 +62,38 @@ __host__ __device__ inline void ideal_tile(int N, int txh_cap, int tz, int &TXH,
 }
 
 // Tile of the single-parity phi march: TXH half-columns x TY rows in x-y,
-// chunks of TZ planes in z (64 x 4 x 16 from N = 128 on: wide rows keep the
-// x-halo columns a small share of the lines each plane touches).
-__host__ __device__ inline void hb_tile(int N, int &TXH, int &TY, int &TZ) {
+// chunks of TZ planes in z.  Defaults (cap, tz = 0): 64 x 4 x 16 from
+// N = 128 on (wide rows keep the x-halo columns a small share of the lines
+// each plane touches); cap and tz are tunables, valid while the packed grid
+// fits (TY+2)(TXH+2) <= kHbMarchGrid and the ring 2 TY + 2 TXH <= kBlock.
+__host__ __device__ inline void hb_tile(int N, int cap, int tz, int &TXH, int &TY, int &TZ) {
   const int NH = N / 2;
-  TXH = NH < 64 ? NH : 64;
+  const int c = cap > 0 ? cap : 64;
+  TXH = NH < c ? NH : c;
   TY = 256 / TXH;
   if (TY > N) TY = N;
-  TZ = N < 16 ? N : 16;
+  const int zc = tz > 0 ? tz : 16;
+  TZ = N < zc ? N : zc;
 }
 
 // Its packed LDS grid: (TY+2)(TXH+2) entries.
 constexpr int kHbMarchGrid = 400;
 
 // Tile of the charge march kernel: TXH half-columns x TQ lines of the
-// cross-section, chunks of C sites along the direction (C even): 16 from
-// N = 128 on, 8 below, where the shorter chunks give small lattices (and
-// batches of them) enough blocks to fill the GPU (measured: 96-replica
-// batch of 32^3-64^3, charge 268 -> 252 us; at 256^3 16 is faster).
-__host__ __device__ inline void charge_tile(int N, int &TXH, int &TQ, int &C) {
+// cross-section, chunks of C sites along the direction (C even).  Defaults
+// (cap, chunk = 0): TXH = min(NH, 64) and chunks of 16 from N = 128 on, 8
+// below, where the shorter chunks give small lattices (and batches of them)
+// enough blocks to fill the GPU (measured: 96-replica batch of 32^3-64^3,
+// charge 268 -> 252 us; at 256^3 16 is faster).  cap and chunk are tunables.
+__host__ __device__ inline void charge_tile(int N, int cap, int chunk, int &TXH, int &TQ,
+                                            int &C) {
   const int NH = N / 2;
-  TXH = NH < 64 ? NH : 64;
+  const int c = cap > 0 ? cap : 64;
+  TXH = NH < c ? NH : c;
   TQ = 256 / TXH;
   if (TQ > N) TQ = N;
-  const int chunk = N >= 128 ? 16 : 8;
-  C = N < chunk ? N : chunk;
+  const int ch = chunk > 0 ? chunk : (N >= 128 ? 16 : 8);
+  C = N < ch ? N : ch;
 }
 
 // Per-block monitor partials (o4_stepper_monitor counters).
@@ -117,6 +126,16 @@ struct Plan {
 // at 64^3-128^3 and for FP32 256^3 (0.7 GB) the plain stores are as fast or
 // faster.
 constexpr uint64_t kStreamBytes = 1ull << 30;
+
+// FP32 fields above this (one buffer, all replicas) also take the phi
+// z-march: its own launch is no faster than the one-thread-per-site kernel
+// (256^3: 110 against 109 us), but it moves 1.09x instead of 1.36x the
+// algorithmic bytes and leaves more of the charge and phi planes in the
+// 256 MB last-level cache for the kernels that follow (256^3 FP32 step
+// 3.72 -> 3.61 ms: charge 337 -> 317 us, ideal 429 -> 414 us).  At 192^3
+// FP64 (540 MiB), 128^3 and the 96-replica FP32 batch the per-site kernel
+// gives the faster step.
+constexpr uint64_t kMarchBytesFp32 = 600ull << 20;
 
 // Largest side: N^3 must fit 32-bit site indices.
 constexpr int kMaxN = 1536;

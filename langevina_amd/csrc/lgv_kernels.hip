@@ -278,7 +278,7 @@ __global__ __launch_bounds__(kBlock) void k_hb_march(const ReplicaParams *__rest
   const int q = ieo ^ 1;
   const int N = p.N, NH = p.NH;
   int TXH, TY, TZ;
-  hb_tile(N, TXH, TY, TZ);
+  hb_tile(N, p.hb_cap, p.hb_tz, TXH, TY, TZ);
   const int ntx = (NH + TXH - 1) / TXH, nty = (N + TY - 1) / TY;
   int b = bid - blk_first[r];
   int bx, by, bz;
@@ -645,7 +645,7 @@ __global__ __launch_bounds__(kBlock) void k_charge_march(const ReplicaParams *__
   const ReplicaState &s = S[r];
   const int N = p.N, NH = p.NH;
   int TXH, TQ, C;
-  charge_tile(N, TXH, TQ, C);
+  charge_tile(N, p.chg_cap, p.chg_chunk, TXH, TQ, C);
   const int nbx = (NH + TXH - 1) / TXH, nbq = (N + TQ - 1) / TQ;
   int b = bid - blk_first[r];
   const int bx = b % nbx;

@@ -58,7 +58,7 @@ EXPORTS = [
     "lgv_init_gaussian_charges", "lgv_thermalize", "lgv_monitor_get", "lgv_energy_get",
     "lgv_rng_get_state", "lgv_rng_set_state", "lgv_measure", "lgv_synchronize",
     "lgv_field_device_ptr", "lgv_timing_enable", "lgv_timing_get", "lgv_measure_full",
-    "lgv_set_mass", "lgv_monitor_reset", "lgv_set_ideal_tile",
+    "lgv_set_mass", "lgv_monitor_reset", "lgv_set_ideal_tile", "lgv_set_sweep_tiles",
     "lgv_em_last_error", "lgv_em_create", "lgv_em_set_stream", "lgv_em_forward",
     "lgv_em_backward", "lgv_em_destroy", "lgv_em_series", "lgv_em_step",
     "lgv_em_step_noalloc",
@@ -111,6 +111,7 @@ def lib():
     L.lgv_set_mass.argtypes = [ctx, C.c_int, C.c_double, C.c_double]
     L.lgv_monitor_reset.argtypes = [ctx]
     L.lgv_set_ideal_tile.argtypes = [ctx, C.c_int, C.c_int, C.c_int]
+    L.lgv_set_sweep_tiles.argtypes = [ctx, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
     L.lgv_synchronize.argtypes = [ctx]
     L.lgv_field_device_ptr.argtypes = [ctx, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_int)]
     _lib = L

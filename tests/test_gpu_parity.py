@@ -442,3 +442,41 @@ def test_64bit_offset_kernels_match(L, orc, monkeypatch):
                 ctx.close()
     for key, val in out.items():
         np.testing.assert_array_equal(val, out["0", "0", key[2]], err_msg=str(key))
+
+
+@pytest.mark.parametrize("N", [48, 128])
+def test_sweep_tiles_do_not_change_results(L, monkeypatch, N):
+    """The tiles of the phi z-march and of the charge march (including tiles
+    that turn the whole-row x sweep of N = 128 off, ragged tiles at N = 48,
+    and short chunks) give the bits and acceptance counts of the defaults;
+    a retile in the middle of a run keeps the monitor counts."""
+    log("sweep tiles", N)
+    from langevina_amd.autotune import set_sweep_tiles
+    monkeypatch.setenv("LGV_PHI_KERNEL", "march")
+    u = np.random.default_rng(N + 1).normal(size=(N, N, N, 10)) * 0.6
+    ref = None
+    for tiles in ((0, 0, 0, 0), (16, 8, 32, 4), (64, 32, 16, 32), (32, 16, 64, 8)):
+        ctx = L.Context(L.model_params(N, seed=9, **PAR))
+        ctx.set_field(u)
+        ctx.run(0.04, 1)
+        set_sweep_tiles(ctx, 0, *tiles)
+        ctx.run(0.04, 1)
+        got, mon = ctx.get_field(), ctx.monitors()
+        ctx.close()
+        if ref is None:
+            ref = (got, mon)
+        else:
+            np.testing.assert_array_equal(got, ref[0], err_msg=str(tiles))
+            for kind in ("phi", "charge"):
+                for k in ("down_counts", "up_yes", "up_no"):
+                    assert mon[kind][k] == ref[1][kind][k], (tiles, kind, k)
+
+
+def test_sweep_tiles_are_checked(L):
+    from langevina_amd.autotune import set_sweep_tiles
+    ctx = L.Context(L.model_params(16, seed=1, **PAR))
+    with pytest.raises(L.LgvError, match="out of range"):
+        set_sweep_tiles(ctx, 0, 0, 0, 16, 5)
+    with pytest.raises(L.LgvError, match="out of range"):
+        set_sweep_tiles(ctx, 0, 2, 0, 0, 0)
+    ctx.close()
