@@ -43,12 +43,12 @@ SCHEDULE_BYTES_PER_SITE_UPDATE_FP32 = SCHEDULE_BYTES_PER_SITE_UPDATE
 # copy-kernel calibration (bench.py --calibrate-copy); the summary of those
 # runs is profiles/modelg_256_hbm_traffic.md.
 MEASURED_TRAFFIC = {
-    ("k_ideal_fused", "fp64", 256): (95.15, 80.00),
-    ("k_hb_march", "fp64", 256): (36.34, 16.01),
-    ("k_charge_march", "fp64", 256): (51.91, 48.23),
-    ("k_ideal_fused", "fp32", 256): (76.76, 40.00),
-    ("k_hb_phi", "fp32", 256): (24.57, 8.12),
-    ("k_charge_march", "fp32", 256): (25.70, 24.14),
+    ("k_ideal_fused", "fp64", 256): (96.32, 80.00),
+    ("k_hb_march", "fp64", 256): (36.51, 16.01),
+    ("k_charge_march", "fp64", 256): (51.94, 48.24),
+    ("k_ideal_fused", "fp32", 256): (76.88, 40.00),
+    ("k_hb_march", "fp32", 256): (17.58, 8.01),
+    ("k_charge_march", "fp32", 256): (25.83, 24.14),
 }
 
 
@@ -326,9 +326,10 @@ def run_single(L, N, fp32, steps, warmup, graph=False, tune=True, ideal_tile=Non
     return out
 
 
-def run_batched(L, fp32, steps, warmup, tune=True):
+def run_batched(L, fp32, steps, warmup, tune=True, graph=True):
     """The 96-replica ensemble in one context, one HIP graph per step, against
-    the same replicas one context (and one graph) at a time."""
+    the same replicas one context (and one graph) at a time; graph=False
+    launches the kernels on the stream instead (for tracing profilers)."""
     from langevina_amd.autotune import (autotune_ideal, autotune_sweeps, set_ideal_tile,
                                         set_sweep_tiles)
     reps = batched_replicas(L)
@@ -346,7 +347,7 @@ def run_batched(L, fp32, steps, warmup, tune=True):
     ctx.run(DT, 5)
     tm = ctx.timing()
     ctx.timing_enable(False)
-    ctx.use_graph(True)
+    ctx.use_graph(graph)
     ctx.run(DT, warmup)
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -363,7 +364,7 @@ def run_batched(L, fp32, steps, warmup, tune=True):
             cap, tz = tuned[p.NX][:2]
             set_ideal_tile(c1, 0, cap, tz)
             set_sweep_tiles(c1, 0, *sweeps[p.NX])
-        c1.use_graph(True)
+        c1.use_graph(graph)
         c1.run(DT, 2)
         c1.synchronize()
         t0 = time.perf_counter()
@@ -391,12 +392,13 @@ def run_batched(L, fp32, steps, warmup, tune=True):
 def bench_batched(args):
     import langevina_amd as L
     b = run_batched(L, args.precision == "fp32", args.steps, args.warmup,
-                    tune=not args.no_autotune)
+                    tune=not args.no_autotune, graph=not args.no_graph)
     out = {"metric": METRIC, "value": b.pop("value"), "unit": "site-updates/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": b.pop("ms_per_step"),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": b.pop("dtype"), "data": "synthetic (phi=0, Gaussian charges per replica)",
-           "config": {"workload": b.pop("workload"), "dt": DT, "parallelism": "1 GPU"}}
+           "config": {"workload": b.pop("workload"), "dt": DT, "parallelism": "1 GPU",
+                      "graph": not args.no_graph}}
     out.update(b)
     print(json.dumps(out), flush=True)
 
@@ -468,6 +470,8 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
                     help="the precision of the headline lattice (fp32: only that form)")
     ap.add_argument("--graph", action="store_true", help="replay a HIP graph per step")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="--batched: launch on the stream instead of replaying graphs")
     ap.add_argument("--batched", action="store_true",
                     help="only the 96 mixed-size replicas in one context")
     ap.add_argument("--headline-only", action="store_true",
