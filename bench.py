@@ -326,10 +326,11 @@ def run_single(L, N, fp32, steps, warmup, graph=False, tune=True, ideal_tile=Non
     return out
 
 
-def run_batched(L, fp32, steps, warmup, tune=True, graph=True):
+def run_batched(L, fp32, steps, warmup, tune=True, graph=True, graph_steps=10):
     """The 96-replica ensemble in one context, one HIP graph per step, against
-    the same replicas one context (and one graph) at a time; graph=False
-    launches the kernels on the stream instead (for tracing profilers)."""
+    the same replicas one context (and one graph) at a time; each graph holds
+    graph_steps steps; graph=False launches the kernels on the stream instead
+    (for tracing profilers)."""
     from langevina_amd.autotune import (autotune_ideal, autotune_sweeps, set_ideal_tile,
                                         set_sweep_tiles)
     reps = batched_replicas(L)
@@ -347,7 +348,7 @@ def run_batched(L, fp32, steps, warmup, tune=True, graph=True):
     ctx.run(DT, 5)
     tm = ctx.timing()
     ctx.timing_enable(False)
-    ctx.use_graph(graph)
+    ctx.use_graph(graph_steps if graph else 0)
     ctx.run(DT, warmup)
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -364,7 +365,7 @@ def run_batched(L, fp32, steps, warmup, tune=True, graph=True):
             cap, tz = tuned[p.NX][:2]
             set_ideal_tile(c1, 0, cap, tz)
             set_sweep_tiles(c1, 0, *sweeps[p.NX])
-        c1.use_graph(graph)
+        c1.use_graph(graph_steps if graph else 0)
         c1.run(DT, 2)
         c1.synchronize()
         t0 = time.perf_counter()
@@ -392,7 +393,8 @@ def run_batched(L, fp32, steps, warmup, tune=True, graph=True):
 def bench_batched(args):
     import langevina_amd as L
     b = run_batched(L, args.precision == "fp32", args.steps, args.warmup,
-                    tune=not args.no_autotune, graph=not args.no_graph)
+                    tune=not args.no_autotune, graph=not args.no_graph,
+                    graph_steps=args.graph_steps)
     out = {"metric": METRIC, "value": b.pop("value"), "unit": "site-updates/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": b.pop("ms_per_step"),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -472,6 +474,8 @@ def main():
     ap.add_argument("--graph", action="store_true", help="replay a HIP graph per step")
     ap.add_argument("--no-graph", action="store_true",
                     help="--batched: launch on the stream instead of replaying graphs")
+    ap.add_argument("--graph-steps", type=int, default=10,
+                    help="steps per captured graph of the batched form")
     ap.add_argument("--batched", action="store_true",
                     help="only the 96 mixed-size replicas in one context")
     ap.add_argument("--headline-only", action="store_true",
@@ -529,7 +533,8 @@ def main():
         f = run_single(L, N, True, args.steps, args.warmup, graph=args.graph,
                        tune=not args.no_autotune)
         progress(f"{f['workload']}: {f['value']:.3e} site-updates/s")
-        b = run_batched(L, True, args.steps, args.warmup, tune=not args.no_autotune)
+        b = run_batched(L, True, args.steps, args.warmup, tune=not args.no_autotune,
+                        graph_steps=args.graph_steps)
         progress(f"batched FP32: {b['value']:.3e} site-updates/s")
         for form in (f, b):
             form["over_fp64"] = form["value"] / h["value"]

@@ -98,7 +98,9 @@ int lgv_step(lgv_ctx *ctx, double dt);
 /* nsteps x { PV2HBSplit::step(dt); t += dt } -- the inner loop of
  * run_event (SuperPions.cxx:124-160) without the measurements. */
 int lgv_run(lgv_ctx *ctx, double dt, int nsteps);
-/* Enables a HIP graph of one full step that lgv_run replays (0 = off). */
+/* HIP graphs for lgv_run: 0 = off (kernels launched on the stream);
+ * K >= 1 = one graph of K full steps (plus one of a single step for the
+ * remainder of a run) captured on first use and replayed. */
 int lgv_use_graph(lgv_ctx *ctx, int enable);
 
 /* The sub-steppers on their own: IdealPV2::step (Stepper.cxx:23-66),

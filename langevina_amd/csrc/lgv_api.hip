@@ -108,7 +108,9 @@ struct lgv_ctx {
   // the current N (see MeasBuf) so that one copy brings them to the host
   double *d_meas = nullptr;
   bool use_graph = false;
-  hipGraphExec_t graph_exec = nullptr;
+  int graph_steps = 1;                 // steps per captured graph
+  hipGraphExec_t graph_exec = nullptr;  // graph_steps steps
+  hipGraphExec_t graph_one = nullptr;   // one step (the remainder of a run)
   double graph_dt = NAN;
   // HIP-event timing of kernel classes on the context stream
   bool timing = false;
@@ -543,6 +545,7 @@ int lgv_destroy(lgv_ctx *c) {
   note(hipSetDevice(c->device));
   if (c->stream) note(hipStreamSynchronize(c->stream));
   if (c->graph_exec) note(hipGraphExecDestroy(c->graph_exec));
+  if (c->graph_one) note(hipGraphExecDestroy(c->graph_one));
   for (hipEvent_t e : c->ev_pool) note(hipEventDestroy(e));
   for (void *f : c->fields) note(hipFree(f));
   note(hipFree(c->dP));
@@ -628,9 +631,25 @@ int lgv_step(lgv_ctx *c, double dt) {
   return check_launch();
 }
 
+// Releases the captured step graphs (they capture plans, tiles and dt).
+static int drop_graphs(lgv_ctx *c) {
+  for (hipGraphExec_t *g : {&c->graph_exec, &c->graph_one}) {
+    if (*g) HIPCHECK(hipGraphExecDestroy(*g));
+    *g = nullptr;
+  }
+  return LGV_OK;
+}
+
 int lgv_use_graph(lgv_ctx *c, int enable) {
   if (!c) return fail(LGV_ERR_ARG, "null context");
+  if (enable < 0 || enable > 1024) return fail(LGV_ERR_ARG, "graph steps %d out of range", enable);
+  HIPCHECK(hipSetDevice(c->device));
   c->use_graph = enable != 0;
+  if (enable > 0 && enable != c->graph_steps) {
+    int rc = drop_graphs(c);
+    if (rc) return rc;
+    c->graph_steps = enable;
+  }
   return LGV_OK;
 }
 
@@ -648,20 +667,27 @@ int lgv_run(lgv_ctx *c, double dt, int nsteps) {
     return check_launch();
   }
   if (!c->graph_exec || c->graph_dt != dt) {
-    if (c->graph_exec) {
-      HIPCHECK(hipGraphExecDestroy(c->graph_exec));
-      c->graph_exec = nullptr;
+    if ((rc = drop_graphs(c))) return rc;
+    // graph_steps steps in one graph, and (when that is more than one) one
+    // step for the remainder of a run
+    for (int g = 0; g < (c->graph_steps > 1 ? 2 : 1); g++) {
+      const int k = g == 0 ? c->graph_steps : 1;
+      hipGraph_t graph;
+      HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      for (int i = 0; i < k; i++) {
+        const int n = enqueue_step(c);
+        launch_advance(c->nrep, c->dS, dt, 1, (uint64_t)n, c->stream);
+      }
+      HIPCHECK(hipStreamEndCapture(c->stream, &graph));
+      HIPCHECK(hipGraphInstantiate(g == 0 ? &c->graph_exec : &c->graph_one, graph, nullptr,
+                                   nullptr, 0));
+      HIPCHECK(hipGraphDestroy(graph));
     }
-    hipGraph_t graph;
-    HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    const int n = enqueue_step(c);
-    launch_advance(c->nrep, c->dS, dt, 1, (uint64_t)n, c->stream);
-    HIPCHECK(hipStreamEndCapture(c->stream, &graph));
-    HIPCHECK(hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0));
-    HIPCHECK(hipGraphDestroy(graph));
     c->graph_dt = dt;
   }
-  for (int i = 0; i < nsteps; i++) HIPCHECK(hipGraphLaunch(c->graph_exec, c->stream));
+  const int K = c->graph_steps;
+  for (int i = 0; i < nsteps / K; i++) HIPCHECK(hipGraphLaunch(c->graph_exec, c->stream));
+  for (int i = 0; i < nsteps % K; i++) HIPCHECK(hipGraphLaunch(c->graph_one, c->stream));
   return check_launch();
 }
 
@@ -950,10 +976,7 @@ int lgv_set_mass(lgv_ctx *c, int r, double mass0, double dmassdt) {
   HIPCHECK(hipStreamSynchronize(c->stream));
   HIPCHECK(hipMemcpy(c->dP, c->hp.data(), c->nrep * sizeof(ReplicaParams),
                      hipMemcpyHostToDevice));
-  if (c->graph_exec) {
-    HIPCHECK(hipGraphExecDestroy(c->graph_exec));
-    c->graph_exec = nullptr;
-  }
+  if (int rg = drop_graphs(c)) return rg;
   return LGV_OK;
 }
 
@@ -978,10 +1001,7 @@ int lgv_set_ideal_tile(lgv_ctx *c, int r, int txh_cap, int tz) {
   HIPCHECK(hipStreamSynchronize(c->stream));
   HIPCHECK(hipMemcpy(c->dP, c->hp.data(), c->nrep * sizeof(ReplicaParams),
                      hipMemcpyHostToDevice));
-  if (c->graph_exec) {
-    HIPCHECK(hipGraphExecDestroy(c->graph_exec));
-    c->graph_exec = nullptr;
-  }
+  if (int rg = drop_graphs(c)) return rg;
   return LGV_OK;
 }
 
@@ -1031,10 +1051,7 @@ int lgv_set_sweep_tiles(lgv_ctx *c, int r, int hb_cap, int hb_tz, int chg_cap,
   HIPCHECK(hipMemcpy(c->dP, c->hp.data(), c->nrep * sizeof(ReplicaParams),
                      hipMemcpyHostToDevice));
   HIPCHECK(hipDeviceSynchronize());
-  if (c->graph_exec) {
-    HIPCHECK(hipGraphExecDestroy(c->graph_exec));
-    c->graph_exec = nullptr;
-  }
+  if (int rg = drop_graphs(c)) return rg;
   return rc;
 }
 

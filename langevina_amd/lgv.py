@@ -231,6 +231,7 @@ class Context:
         _check(lib().lgv_run(self.h, dt, nsteps))
 
     def use_graph(self, enable=True):
+        """False: stream launches; True or K: replay a graph of K steps."""
         _check(lib().lgv_use_graph(self.h, int(enable)))
 
     def ideal_step(self, dt):

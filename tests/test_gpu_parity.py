@@ -480,3 +480,22 @@ def test_sweep_tiles_are_checked(L):
     with pytest.raises(L.LgvError, match="out of range"):
         set_sweep_tiles(ctx, 0, 2, 0, 0, 0)
     ctx.close()
+
+
+def test_multi_step_graph_matches_stream(L, orc):
+    """A graph of 4 steps replayed for a run of 11 (two replays and three
+    one-step graphs) gives the bits, clock and counters of stream launches."""
+    log("multi-step graph")
+    N = 16
+    a, _ = make_pair(L, orc, N, seed=15)
+    b, _ = make_pair(L, orc, N, seed=15)
+    b.use_graph(4)
+    for n in (11, 4, 1):
+        a.run(0.04, n)
+        b.run(0.04, n)
+        np.testing.assert_array_equal(a.get_field(), b.get_field(), err_msg=f"run of {n}")
+        assert a.time() == b.time()
+        assert a.rng_state().sweep_ctr == b.rng_state().sweep_ctr
+    assert a.monitors() == b.monitors()
+    a.close()
+    b.close()
