@@ -185,7 +185,7 @@ int build_ideal_plan(lgv_ctx *c) {
     const ReplicaParams &p = c->hp[r];
     int TXH, TY, TZ;
     ideal_tile(p.N, p.tile_cap, p.tile_tz, TXH, TY, TZ);
-    if ((TY + 2) * (2 * TXH + 2) > kIdealLds)
+    if ((TY + 2) * (2 * TXH + 2) > ideal_lds(c->fp32))
       return fail(LGV_ERR_ARG, "replica %d: ideal tile %dx%d exceeds the LDS plane", r, TXH, TY);
     work.push_back((int64_t)((p.NH + TXH - 1) / TXH) * ((p.N + TY - 1) / TY) *
                    ((p.N + TZ - 1) / TZ) * kBlock);

@@ -50,6 +50,11 @@ struct ReplicaState {
 // one-site halo ring (2 TY + 4 TXH sites) may exceed the block's threads
 // (64 half-columns x 4 rows: 264); its remainder then takes a second pass.
 constexpr int kIdealLds = 800;
+// FP32 fields keep the raw charges of the current plane in LDS as well, so
+// their plane grid is held to the tiles of width <= 32 half-columns (at most
+// 660 entries) and three blocks still fit one CU's LDS.
+constexpr int kIdealLdsF32 = 660;
+__host__ __device__ constexpr int ideal_lds(bool fp32) { return fp32 ? kIdealLdsF32 : kIdealLds; }
 __host__ __device__ inline void ideal_tile(int N, int txh_cap, int tz, int &TXH, int &TY,
                                            int &TZ) {
   const int NH = N / 2;

@@ -717,7 +717,7 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
                                                         const ReplicaState *__restrict__ S,
                                                         const int *__restrict__ blk_rep,
                                                         const int *__restrict__ blk_first) {
-  __shared__ double lds[4][kIdealLds];
+  __shared__ double lds[4][ideal_lds(sizeof(T) == 4)];
   // rotated phi of plane z-1 at the thread's own sites: thread-private slots
   __shared__ double lprv[2][4][kBlock];
   const int bid = xcd_block();
@@ -767,11 +767,14 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
   const double h = 0.5 * (p.tau_ideal / p.chi);
   const double axx = p.axx, H0 = p.H, dt = p.dt_ideal;
 
-  // FP64 fields carry the raw charges of planes z and z+1 in registers; FP32
-  // fields re-read those of plane z at kick time, which keeps the kernel at
-  // three waves per SIMD (measured faster for FP32, slower for FP64, whose
-  // re-reads miss L2)
+  // FP64 fields carry the raw charges of planes z and z+1 in registers (two
+  // waves per SIMD).  FP32 fields keep those of plane z in thread-private
+  // LDS slots (as stored, in FP32) and those of plane z+1 in registers from
+  // their load to the end of the iteration, which keeps three waves per
+  // SIMD; re-reading plane z's charges at kick time instead made the kernel
+  // read 1.92x its algorithmic bytes (the re-reads miss the 4 MB L2).
   constexpr bool kCarry = sizeof(T) == 8;
+  __shared__ T chq[kCarry ? 1 : 2][kCarry ? 1 : 6][kCarry ? 1 : kBlock];
   // FP64 fields also load the raw fields of the thread's first ring site of
   // plane z+1 before the kick of plane z, so that their latency overlaps the
   // kick (free at FP64's 2 waves per SIMD; FP32 would drop from 3 waves to 2)
@@ -782,8 +785,9 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
   // sits at x = 2 xh + ((pp + y + z) & 1); its z +- 1 neighbours have parity
   // pp ^ 1 in the same half-column.
   double cen[2][4], nex[2][4];  // rotated phi at the own sites of planes z, z+1
-  double chg[2][6], chn[2][6];  // raw charges of planes z, z+1 (kCarry only)
-  auto rotate_own = [&](int z, double out[2][4], double ch[2][6]) __attribute__((always_inline)) {
+  double chg[2][6];             // raw charges of plane z (kCarry only)
+  T chn[2][6];                  // raw charges of plane z+1, as stored
+  auto rotate_own = [&](int z, double out[2][4], T ch[2][6]) __attribute__((always_inline)) {
     if (!own) return;
     const uint32_t i = ((uint32_t)z * N + y) * NH + xh;
 #pragma unroll
@@ -794,10 +798,8 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
       rotate_site_fast(f, f + 4, f + 7, h);
 #pragma unroll
       for (int c = 0; c < 4; c++) out[pp][c] = f[c];
-      if constexpr (kCarry) {
 #pragma unroll
-        for (int c = 0; c < 6; c++) ch[pp][c] = f[4 + c];
-      }
+      for (int c = 0; c < 6; c++) ch[pp][c] = (T)f[4 + c];
     }
   };
   auto fill_lds = [&](int z, const double o[2][4]) __attribute__((always_inline)) {
@@ -831,7 +833,17 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
 #pragma unroll
       for (int c = 0; c < 4; c++) lprv[s][c][t] = nex[s][c];
   }
-  rotate_own(z0, cen, chg);
+  rotate_own(z0, cen, chn);
+  // the charges of plane z0: registers (FP64) or the thread's LDS slots (FP32)
+  if (own) {
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+      for (int c = 0; c < 6; c++) {
+        if constexpr (kCarry) chg[s][c] = (double)chn[s][c];
+        else chq[s][c][t] = chn[s][c];
+      }
+  }
   if (kEarlyRing && t < nh) {
     double f[10];
     load_site<T, W>(p, cur, rgx, rgy, z0, f);
@@ -876,8 +888,8 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
             A[l] = chg[pp][l];
             V[l] = chg[pp][3 + l];
           } else {
-            A[l] = gplane<T, W>(p, cur, 4 + l, par).ld(i);
-            V[l] = gplane<T, W>(p, cur, 7 + l, par).ld(i);
+            A[l] = (double)chq[pp][l][t];
+            V[l] = (double)chq[pp][3 + l][t];
           }
         }
 #pragma unroll
@@ -924,9 +936,12 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
         if (own) lprv[s][c][t] = cen[s][c];
         cen[s][c] = nex[s][c];
       }
-      if constexpr (kCarry) {
+      if (own) {
 #pragma unroll
-        for (int c = 0; c < 6; c++) chg[s][c] = chn[s][c];
+        for (int c = 0; c < 6; c++) {
+          if constexpr (kCarry) chg[s][c] = (double)chn[s][c];
+          else chq[s][c][t] = chn[s][c];
+        }
       }
     }
   }
