@@ -4,7 +4,7 @@ for them (nothing forced):
 * 256^3 FP64: the phi z-march with non-temporal loads and stores, the
   ideal kernel in y-band block order, the charge sweep with whole x-rows
   per wavefront -- one full PV2HBSplit23 step against the oracle (its loops
-  split over host threads);
+  split over host threads), then the measurement pass on the result;
 * 256^3 FP32 fields: one full step against the oracle's FP32 mode;
 * a batch of FP32 replicas of mixed sizes, own seeds and H drawn from a
   xoroshiro128+ stream, stepped through a replayed HIP graph.
@@ -90,6 +90,19 @@ def test_256_cubed_full_step(L, orc, threads, precision):
         assert (ul > 1).sum() <= 1e-6 * ul.size
     assert _counts(ctx.monitors()) == _counts(lat.monitors())
     assert ctx.rng_state().sweep_ctr == lat.c.sweep_ctr
+    if precision == 0:
+        # the measurement at full size: walls, phase walls, OAverage and the
+        # r2c rows of the device pass against the oracle's on its own field
+        log("measuring")
+        m = ctx.measure_full()
+        w, wp, oa = orc.measure_walls(want)
+        np.testing.assert_allclose(m["oavg"], oa, rtol=1e-10, atol=1e-14)
+        np.testing.assert_allclose(m["wall"], w, rtol=1e-10, atol=1e-13)
+        np.testing.assert_allclose(m["wallphase"], wp, rtol=1e-10, atol=1e-13)
+        for d in range(3):
+            k = orc.dft_rows(w[d])
+            np.testing.assert_allclose(m["wall_k"][d], k[..., 0] + 1j * k[..., 1], rtol=1e-9,
+                                       atol=1e-13)
     ctx.close()
 
 
