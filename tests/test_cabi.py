@@ -104,3 +104,33 @@ def test_em_create_rejects_bad_arguments(L):
     p.noise_kind = 3
     assert lib.lgv_em_create(C.byref(p), 10, 5, 0, C.byref(h)) == -1
     assert b"noise_kind" in lib.lgv_em_last_error()
+
+
+def test_em_step_entry_points_reject_bad_arguments(L):
+    """The single-step entry points check their arguments before any device
+    call: null handle or buffers, in == out, a negative step, and (the entry
+    point without allocation) a lattice beyond its static scratch array."""
+    from langevina_amd.em import EmParams, _bind
+    lib = _bind()
+    buf = (C.c_double * 8)()
+    a, b = C.cast(buf, C.c_void_p), C.c_void_p(C.addressof(buf) + 32)
+    assert lib.lgv_em_step(None, a, b, 0, None) == -1
+    assert lib.lgv_em_series(None, a, 10, None, a) == -1
+    p = EmParams()
+    p.N, p.dt, p.gamma, p.noise_kind = 258, 0.01, 1.0, 0
+    assert lib.lgv_em_step_noalloc(C.byref(p), a, b, 0, None, 0, None) == -4
+    assert b"N=258" in lib.lgv_em_last_error()
+    p.N = 16
+    assert lib.lgv_em_step_noalloc(C.byref(p), a, a, 0, None, 0, None) == -1
+    assert b"differ" in lib.lgv_em_last_error()
+    assert lib.lgv_em_step_noalloc(C.byref(p), a, b, -1, None, 0, None) == -1
+    p.noise_kind = 5
+    assert lib.lgv_em_step_noalloc(C.byref(p), a, b, 0, None, 0, None) == -1
+    assert lib.lgv_em_step_noalloc(None, a, b, 0, None, 0, None) == -1
+
+
+def test_tile_and_graph_setters_reject_a_null_context(L):
+    lib = L.lib()
+    assert lib.lgv_set_sweep_tiles(None, 0, 0, 0, 0, 0) == -1
+    assert lib.lgv_set_ideal_tile(None, 0, 32, 16) == -1
+    assert lib.lgv_use_graph(None, 1) == -1
