@@ -81,6 +81,11 @@ struct HostNoise {
 struct lgv_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // the charge sweep of a step runs on stream2, concurrently with the phi
+  // sweeps that follow the last ideal substep (see enqueue_step)
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool overlap = true;
   bool fp32 = false;
   unsigned s0 = 2, s1 = 3;
   bool nodiffuse = false, onlydiffuse = false, track_energy = false;
@@ -365,23 +370,38 @@ void enqueue_hb(lgv_ctx *c, int off) {
 
 // ModelGChargeHB::step: the shuffle, then per direction both parities in
 // one pass (counters off+1+2*ixyz and off+2+2*ixyz, as the unfused order).
-void enqueue_charge(lgv_ctx *c, int off) {
-  launch_shuffle(c->nrep, c->dS, c->dP, off, c->stream);
+void enqueue_charge(lgv_ctx *c, int off, hipStream_t st) {
+  launch_shuffle(c->nrep, c->dS, c->dP, off, st);
   for (int ixyz = 0; ixyz < 3; ixyz++) {
     Timed tm(c, LGV_CLASS_CHARGE);
     launch_charge_march(c->fp32, c->plan_charge, c->dP, c->dS, c->nrep, ixyz,
-                        off + 1 + 2 * ixyz, c->mon_charge, c->stream);
+                        off + 1 + 2 * ixyz, c->mon_charge, st);
   }
 }
 
 // PV2HBSplit::step (Stepper.cxx:542-580); returns the number of stream
-// counters consumed.
+// counters consumed.  The charge sweep reads and writes only the charges
+// and the phi sweeps only phi, so after the last ideal substep the charge
+// sweep runs on the second stream while the phi sweeps run on the first
+// (fork and join by events, which a graph capture records as well).  Every
+// sweep keeps its counter (random streams) and its monitor slots, so the
+// result is the sequential one bit for bit; the charge sweep is bound by its
+// arithmetic and the phi sweeps by memory, so the two overlap.  Timing runs
+// stay sequential (events on one stream).
 int enqueue_step(lgv_ctx *c) {
   int off = 0;
+  bool charge_queued = false, join_event = false;
+  const bool fork = c->overlap && !c->timing && !c->onlydiffuse && !c->nodiffuse;
   if (!c->onlydiffuse) {
     for (unsigned i1 = 0; i1 < c->s1; i1++) {
       enqueue_ideal(c, off);
       off += 1;
+      if (fork && i1 + 1 == c->s1 && hipEventRecord(c->ev_fork, c->stream) == hipSuccess &&
+          hipStreamWaitEvent(c->stream2, c->ev_fork, 0) == hipSuccess) {
+        enqueue_charge(c, off + 2 * (int)c->s0, c->stream2);
+        charge_queued = true;
+        join_event = hipEventRecord(c->ev_join, c->stream2) == hipSuccess;
+      }
       for (unsigned i0 = 0; i0 < c->s0; i0++) {
         enqueue_hb(c, off);
         off += 2;
@@ -389,7 +409,11 @@ int enqueue_step(lgv_ctx *c) {
     }
   }
   if (!c->nodiffuse) {
-    enqueue_charge(c, off);
+    if (!charge_queued) {
+      enqueue_charge(c, off, c->stream);
+    } else if (!join_event || hipStreamWaitEvent(c->stream, c->ev_join, 0) != hipSuccess) {
+      (void)hipStreamSynchronize(c->stream2);  // join on the host instead
+    }
     off += 7;
   }
   return off;
@@ -439,6 +463,11 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
   c->device = cfg->device;
   HIPCHECK(hipSetDevice(c->device));
   HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPCHECK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+  HIPCHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+  HIPCHECK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+  // test hook: LGV_OVERLAP=0 runs the charge sweep after the phi sweeps
+  if (const char *ov = getenv("LGV_OVERLAP")) c->overlap = atoi(ov) != 0;
   c->fp32 = cfg->precision == LGV_FP32;
   c->s0 = s0;
   c->s1 = s1;
@@ -558,6 +587,10 @@ int lgv_destroy(lgv_ctx *c) {
                   (void *)c->eslots, (void *)c->d_aos, (void *)c->d_partXZ, (void *)c->d_partXY,
                   (void *)c->d_meas})
     note(hipFree(q));
+  if (c->stream2) note(hipStreamSynchronize(c->stream2));
+  if (c->ev_fork) note(hipEventDestroy(c->ev_fork));
+  if (c->ev_join) note(hipEventDestroy(c->ev_join));
+  if (c->stream2) note(hipStreamDestroy(c->stream2));
   if (c->stream) note(hipStreamDestroy(c->stream));
   delete c;
   if (first != hipSuccess)
@@ -716,7 +749,7 @@ int lgv_charge_step(lgv_ctx *c, double dt) {
   HIPCHECK(hipSetDevice(c->device));
   int rc = ensure_coeffs(c, c->coef_hb, dt, c->coef_ideal);
   if (rc) return rc;
-  enqueue_charge(c, 0);
+  enqueue_charge(c, 0, c->stream);
   launch_advance(c->nrep, c->dS, 0., 0, 7, c->stream);
   return check_launch();
 }

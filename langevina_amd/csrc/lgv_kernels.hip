@@ -232,7 +232,7 @@ __global__ __launch_bounds__(kBlock) void k_hb_phi(const ReplicaParams *__restri
                             (z * N + ym) * NH + xh, (zp * N + y) * NH + xh,
                             (zm * N + y) * NH + xh};
     const ReplicaState &s = S[r];
-    const int cur = live(s);
+    const int cur = s.cur_phi;  // phi planes only (the charge bit may be flipping)
     const double mass = p.mass0 + p.dmassdt * s.t;
     const double Lambda2 = 0.5 * (2. * 3. + mass);
     Xoro g = xoro_seeded(site_key(stream_key(p.base_key, s.sweep_ctr + ctr_off),
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(kBlock) void k_hb_march(const ReplicaParams *__rest
   const int r = blk_rep[bid];
   const ReplicaParams &p = P[r];
   const ReplicaState &s = S[r];
-  const int cur = live(s);
+  const int cur = s.cur_phi;  // phi planes only (the charge bit may be flipping)
   const int q = ieo ^ 1;
   const int N = p.N, NH = p.NH;
   int TXH, TY, TZ;

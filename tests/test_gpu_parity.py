@@ -499,3 +499,32 @@ def test_multi_step_graph_matches_stream(L, orc):
     assert a.monitors() == b.monitors()
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("N,prec", [(16, 0), (32, 1)])
+def test_concurrent_charge_sweep_matches_sequential(L, orc, monkeypatch, N, prec):
+    """The charge sweep on the second stream, concurrent with the last phi
+    sweeps of the step, gives the bits, counters and monitors of the
+    sequential order, on the stream and through a graph; and the oracle's."""
+    log("overlap", N, prec)
+    out = {}
+    for ov in ("0", "1"):
+        monkeypatch.setenv("LGV_OVERLAP", ov)
+        for graph in (0, 3):
+            ctx, lat = make_pair(L, orc, N, seed=19, precision=prec, scale=0.6)
+            ctx.use_graph(graph)
+            ctx.run(0.04, 4)
+            out[ov, graph] = (ctx.get_field(), ctx.monitors(), ctx.rng_state().sweep_ctr)
+            ctx.close()
+    ref = out["0", 0]
+    for key, (f, m, ctr) in out.items():
+        np.testing.assert_array_equal(f, ref[0], err_msg=str(key))
+        assert m == ref[1] and ctr == ref[2], key
+    for _ in range(4):
+        lat.step(0.04)
+        lat.t = lat.t + 0.04
+    if prec == 0:
+        assert rel_err(ref[0], lat.u) <= 1e-12
+    else:
+        ulp = np.spacing(np.abs(lat.u).astype(np.float32)).astype(np.float64)
+        assert (np.abs(ref[0] - lat.u) <= 4 * ulp).all()
