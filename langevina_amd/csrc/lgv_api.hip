@@ -573,6 +573,7 @@ int lgv_destroy(lgv_ctx *c) {
   };
   note(hipSetDevice(c->device));
   if (c->stream) note(hipStreamSynchronize(c->stream));
+  if (c->stream2) note(hipStreamSynchronize(c->stream2));
   if (c->graph_exec) note(hipGraphExecDestroy(c->graph_exec));
   if (c->graph_one) note(hipGraphExecDestroy(c->graph_one));
   for (hipEvent_t e : c->ev_pool) note(hipEventDestroy(e));
@@ -587,7 +588,6 @@ int lgv_destroy(lgv_ctx *c) {
                   (void *)c->eslots, (void *)c->d_aos, (void *)c->d_partXZ, (void *)c->d_partXY,
                   (void *)c->d_meas})
     note(hipFree(q));
-  if (c->stream2) note(hipStreamSynchronize(c->stream2));
   if (c->ev_fork) note(hipEventDestroy(c->ev_fork));
   if (c->ev_join) note(hipEventDestroy(c->ev_join));
   if (c->stream2) note(hipStreamDestroy(c->stream2));
