@@ -346,16 +346,14 @@ void enqueue_ideal(lgv_ctx *c, int off) {
 }
 
 // EulerLangevinHB::step (counters off, off+1 for the two parities), one
-// in-place launch per parity.  FP64 fields: the z-march with the other
-// parity's plane in LDS (less traffic); FP32 fields: one thread per site
-// (more waves in flight), each the faster one for its storage on MI355X.
-// The phi half-sweep kernel: the z-march for FP64 fields in the large-field
-// (non-temporal) mode, the one-thread-per-site kernel otherwise.  Measured
-// per half-sweep, FP64: 256^3 march 157 us against 204 us per site; 192^3
-// 85 against 75 (70 in plane bands), 128^3 35 against 29 (26), 64^3 30
-// against 9 (below 1 GiB the
-// fields stay largely cache-resident and the per-site kernel's 8 waves per
-// SIMD win); FP32 256^3 114 against 111.
+// in-place launch per parity: the z-march with the other parity's plane in
+// LDS for FP64 fields above 1 GiB (non-temporal mode) and FP32 fields above
+// 600 MiB, the one-thread-per-site kernel otherwise.  Measured per
+// half-sweep, FP64: 256^3 march 157 us against 204 us per site; 192^3 85
+// against 75 (70 in plane bands), 128^3 35 against 29 (26), 64^3 30 against
+// 9 (below 1 GiB the fields stay largely cache-resident and the per-site
+// kernel's 8 waves per SIMD win); FP32 256^3 110 against 109, but the
+// march's lower traffic speeds up the kernels that follow (kMarchBytesFp32).
 bool use_march(const lgv_ctx *c) { return c->phi_march; }
 const Plan &hb_plan(const lgv_ctx *c) { return use_march(c) ? c->plan_hb : c->plan_half; }
 void enqueue_hb(lgv_ctx *c, int off) {
