@@ -13,9 +13,11 @@ import os
 from .lgv import _check, lib
 
 CAPS = (8, 16, 32, 64)
-TZS = (4, 8, 16, 32, 64)
+# short z chunks give a small lattice (one 64^3 replica: 32 blocks with the
+# default tile) more blocks than the march's depth costs it
+TZS = (2, 4, 8, 16, 32, 64)
 CHARGE_CAPS = (16, 32, 64)
-CHARGE_CHUNKS = (4, 8, 16, 32)
+CHARGE_CHUNKS = (2, 4, 8, 16, 32)
 HB_CAPS = (16, 32, 64)
 HB_TZS = (8, 16, 32)
 

@@ -224,7 +224,7 @@ def test_ideal_tile_does_not_change_results(L, orc, N):
     from langevina_amd.autotune import set_ideal_tile
     u = np.random.default_rng(9).normal(size=(N, N, N, 10)) * 0.5
     ref = None
-    for cap, tz in ((32, 16), (8, 4), (16, 32), (64, 8)):
+    for cap, tz in ((32, 16), (8, 4), (16, 32), (64, 8), (16, 2), (8, 1)):
         ctx = L.Context(L.model_params(N, seed=3, **PAR))
         set_ideal_tile(ctx, 0, cap, tz)
         ctx.set_field(u)
@@ -455,7 +455,8 @@ def test_sweep_tiles_do_not_change_results(L, monkeypatch, N):
     monkeypatch.setenv("LGV_PHI_KERNEL", "march")
     u = np.random.default_rng(N + 1).normal(size=(N, N, N, 10)) * 0.6
     ref = None
-    for tiles in ((0, 0, 0, 0), (16, 8, 32, 4), (64, 32, 16, 32), (32, 16, 64, 8)):
+    for tiles in ((0, 0, 0, 0), (16, 8, 32, 4), (64, 32, 16, 32), (32, 16, 64, 8),
+                  (16, 4, 16, 2)):
         ctx = L.Context(L.model_params(N, seed=9, **PAR))
         ctx.set_field(u)
         ctx.run(0.04, 1)
