@@ -269,7 +269,9 @@ __global__ __launch_bounds__(kBlock) void k_hb_march(const ReplicaParams *__rest
                                                      const int *__restrict__ blk_first, int ieo,
                                                      int ctr_off, MonSlot *__restrict__ mon) {
   constexpr int kGrid = kHbMarchGrid;
-  __shared__ T Q[2][4][kGrid];
+  // values held widened to FP64 (exact for FP32 fields): each is read by up
+  // to four sites, so one conversion on the way in replaces four on the way out
+  __shared__ double Q[2][4][kGrid];
   const int bid = xcd_block();
   const int r = blk_rep[bid];
   const ReplicaParams &p = P[r];
@@ -328,7 +330,7 @@ __global__ __launch_bounds__(kBlock) void k_hb_march(const ReplicaParams *__rest
   for (int j = 0; j < tz; j++) {
     const int z = z0 + j;
     const int zp = z + 1 == N ? 0 : z + 1;
-    T(*g)[kGrid] = Q[j & 1];
+    double(*g)[kGrid] = Q[j & 1];
     double phi_o[4];
     if (own) {
       ld4(q, idx(xh, y, zp), qp);
@@ -338,13 +340,13 @@ __global__ __launch_bounds__(kBlock) void k_hb_march(const ReplicaParams *__rest
         phi_o[l] = gplane<T, W>(p, cur, l, ieo).template get<NTS>(idx(xh, y, z));
       const int pos = (ty + 1) * GW + (tx + 1);
 #pragma unroll
-      for (int l = 0; l < 4; l++) g[l][pos] = (T)qc[l];
+      for (int l = 0; l < 4; l++) g[l][pos] = qc[l];
     }
     if (halo) {
       double v[4];
       ld4(q, idx(hgxh, hgy, z), v);
 #pragma unroll
-      for (int l = 0; l < 4; l++) g[l][hpos] = (T)v[l];
+      for (int l = 0; l < 4; l++) g[l][hpos] = v[l];
     }
     __syncthreads();
     if (own) {
@@ -353,8 +355,8 @@ __global__ __launch_bounds__(kBlock) void k_hb_march(const ReplicaParams *__rest
       double heff[4], phi_n[4];
 #pragma unroll
       for (int l = 0; l < 4; l++)
-        heff[l] = ((double)g[l][pos + o] + (double)g[l][pos - 1 + o]) +
-                  ((double)g[l][pos + GW] + (double)g[l][pos - GW]) + (qp[l] + qm[l]);
+        heff[l] = (g[l][pos + o] + g[l][pos - 1 + o]) +
+                  (g[l][pos + GW] + g[l][pos - GW]) + (qp[l] + qm[l]);
       Xoro gg = xoro_seeded(site_key(key, ((uint64_t)z * N + y) * N + 2 * xh + o));
       hb_decide(phi_o, heff, Lambda2, lambda, H, rdtg, gg, phi_n, cnt, dSd, dSu, true);
       const uint32_t i = idx(xh, y, z);
