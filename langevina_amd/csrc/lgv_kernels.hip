@@ -41,29 +41,36 @@ struct GPlane {
     if constexpr (W) return (uint64_t)i * sizeof(T);
     else return (uint32_t)(i * (uint32_t)sizeof(T));
   }
-  __device__ __forceinline__ double ld(uint32_t i) const {
-    return (double)*reinterpret_cast<const __attribute__((address_space(1))) T *>(b + off(i));
+  __device__ __forceinline__ T raw(uint32_t i) const {
+    return *reinterpret_cast<const __attribute__((address_space(1))) T *>(b + off(i));
   }
+  __device__ __forceinline__ double ld(uint32_t i) const { return (double)raw(i); }
   __device__ __forceinline__ void st(uint32_t i, double v) const {
     *reinterpret_cast<__attribute__((address_space(1))) T *>(b + off(i)) = (T)v;
   }
   // NTS: a non-temporal load, for a value no other thread reads
   template <bool NTS>
-  __device__ __forceinline__ double get(uint32_t i) const {
+  __device__ __forceinline__ T get_raw(uint32_t i) const {
     if constexpr (NTS)
-      return (double)__builtin_nontemporal_load(
+      return __builtin_nontemporal_load(
           reinterpret_cast<const __attribute__((address_space(1))) T *>(b + off(i)));
     else
-      return ld(i);
+      return raw(i);
   }
+  template <bool NTS>
+  __device__ __forceinline__ double get(uint32_t i) const { return (double)get_raw<NTS>(i); }
   // NTS: a non-temporal store (not kept in the caches)
   template <bool NTS>
-  __device__ __forceinline__ void put(uint32_t i, double v) const {
+  __device__ __forceinline__ void put_raw(uint32_t i, T v) const {
     if constexpr (NTS)
       __builtin_nontemporal_store(
-          (T)v, reinterpret_cast<__attribute__((address_space(1))) T *>(b + off(i)));
+          v, reinterpret_cast<__attribute__((address_space(1))) T *>(b + off(i)));
     else
-      st(i, v);
+      *reinterpret_cast<__attribute__((address_space(1))) T *>(b + off(i)) = v;
+  }
+  template <bool NTS>
+  __device__ __forceinline__ void put(uint32_t i, double v) const {
+    put_raw<NTS>(i, (T)v);
   }
 };
 template <typename T, bool W>
@@ -318,36 +325,78 @@ __global__ __launch_bounds__(kBlock) void k_hb_march(const ReplicaParams *__rest
   auto idx = [&](int xhh, int yy, int z) __attribute__((always_inline)) {
     return ((uint32_t)z * N + yy) * NH + xhh;
   };
-  auto ld4 = [&](int par, uint32_t i, double v[4]) __attribute__((always_inline)) {
+  // the eight plane bases, formed once (a store would otherwise force
+  // p.fields to be reloaded)
+  GPlane<T, W> gq[4], go[4];
 #pragma unroll
-    for (int l = 0; l < 4; l++) v[l] = gplane<T, W>(p, cur, l, par).ld(i);
+  for (int l = 0; l < 4; l++) {
+    gq[l] = gplane<T, W>(p, cur, l, q);
+    go[l] = gplane<T, W>(p, cur, l, ieo);
+  }
+  auto ld4 = [&](uint32_t i, double v[4]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int l = 0; l < 4; l++) v[l] = gq[l].ld(i);
+  };
+  // The loads of plane z+1 (the own half-column's q values at z+2, its own
+  // update-parity value at z+1, the ring entry at z+1) are issued before
+  // plane z's barrier, into the registers nq, nphi, nh as stored, so that
+  // they are in flight during plane z's decisions.
+  auto raw4 = [&](uint32_t i, T v[4]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int l = 0; l < 4; l++) v[l] = gq[l].raw(i);
+  };
+  // the update parity's own value: read by this thread only
+  auto own4 = [&](uint32_t i, T v[4]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int l = 0; l < 4; l++) v[l] = go[l].template get_raw<NTS>(i);
+  };
+  // Plane z's new values are stored in iteration z+1, after the loads of
+  // plane z+2 are issued and before the barrier: one in-order counter covers
+  // loads and stores, so the next iteration's wait for those loads also
+  // waits for these stores, which have had the decisions of plane z+1 to
+  // complete (a store at the end of an iteration would be waited for at
+  // once).  Only this thread reads its own sites.
+  auto store4 = [&](uint32_t i, const T v[4]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int l = 0; l < 4; l++) go[l].template put_raw<NTS>(i, v[l]);
   };
   double qm[4], qc[4], qp[4];  // q values of the own half-column at z-1, z, z+1
+  T nq[4], nphi[4], nh[4];
+  T pend[4];         // plane z-1's new values, stored at the top of iteration z
+  uint32_t pend_i = 0;
   if (own) {
-    ld4(q, idx(xh, y, wrapz(z0 - 1, N)), qm);
-    ld4(q, idx(xh, y, z0), qc);
+    ld4(idx(xh, y, wrapz(z0 - 1, N)), qm);
+    ld4(idx(xh, y, z0), qc);
+    raw4(idx(xh, y, z0 + 1 == N ? 0 : z0 + 1), nq);
+    own4(idx(xh, y, z0), nphi);
   }
+  if (halo) raw4(idx(hgxh, hgy, z0), nh);
   for (int j = 0; j < tz; j++) {
     const int z = z0 + j;
     const int zp = z + 1 == N ? 0 : z + 1;
     double(*g)[kGrid] = Q[j & 1];
     double phi_o[4];
     if (own) {
-      ld4(q, idx(xh, y, zp), qp);
-      // the update parity's own value: read by this thread only
-#pragma unroll
-      for (int l = 0; l < 4; l++)
-        phi_o[l] = gplane<T, W>(p, cur, l, ieo).template get<NTS>(idx(xh, y, z));
       const int pos = (ty + 1) * GW + (tx + 1);
 #pragma unroll
-      for (int l = 0; l < 4; l++) g[l][pos] = qc[l];
+      for (int l = 0; l < 4; l++) {
+        qp[l] = nq[l];
+        phi_o[l] = nphi[l];
+        g[l][pos] = qc[l];
+      }
     }
     if (halo) {
-      double v[4];
-      ld4(q, idx(hgxh, hgy, z), v);
 #pragma unroll
-      for (int l = 0; l < 4; l++) g[l][hpos] = v[l];
+      for (int l = 0; l < 4; l++) g[l][hpos] = nh[l];
     }
+    if (j + 1 < tz) {  // plane z+1 (z+1 < z0 + tz <= N)
+      if (own) {
+        raw4(idx(xh, y, zp + 1 == N ? 0 : zp + 1), nq);
+        own4(idx(xh, y, zp), nphi);
+      }
+      if (halo) raw4(idx(hgxh, hgy, zp), nh);
+    }
+    if (own && j > 0) store4(pend_i, pend);
     __syncthreads();
     if (own) {
       const int o = (ieo + y + z) & 1;  // x = 2 xh + o
@@ -359,16 +408,16 @@ __global__ __launch_bounds__(kBlock) void k_hb_march(const ReplicaParams *__rest
                   (g[l][pos + GW] + g[l][pos - GW]) + (qp[l] + qm[l]);
       Xoro gg = xoro_seeded(site_key(key, ((uint64_t)z * N + y) * N + 2 * xh + o));
       hb_decide(phi_o, heff, Lambda2, lambda, H, rdtg, gg, phi_n, cnt, dSd, dSu, true);
-      const uint32_t i = idx(xh, y, z);
-#pragma unroll
-      for (int l = 0; l < 4; l++) gplane<T, W>(p, cur, l, ieo).template put<NTS>(i, phi_n[l]);
+      pend_i = idx(xh, y, z);
 #pragma unroll
       for (int l = 0; l < 4; l++) {
+        pend[l] = (T)phi_n[l];
         qm[l] = qc[l];
         qc[l] = qp[l];
       }
     }
   }
+  if (own && tz > 0) store4(pend_i, pend);
   block_monitor_add(mon + bid, cnt, dSd, dSu);
 }
 
