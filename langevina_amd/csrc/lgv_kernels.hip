@@ -621,7 +621,7 @@ __device__ __forceinline__ void charge_row_wave(const ReplicaParams &p, int cur,
 //    pairs whose lower site lies in it (the monitor counts those).
 //  * d = x: pairs lie in rows; the thread runs charge_line for the row
 //    (y, z) of each z of its chunk.
-template <typename T, bool W>
+template <typename T, bool W, bool PIPE>
 __device__ __forceinline__ void charge_column(const ReplicaParams &p, int cur, int nxt, int dir,
                                               int e1, int e2, uint32_t x, uint32_t q, int c0,
                                               int c1, uint64_t k1, uint64_t k2, uint64_t &cnt,
@@ -637,15 +637,48 @@ __device__ __forceinline__ void charge_column(const ReplicaParams &p, int cur, i
   };
   // lower site of the first first-parity pair: c0-1 or c0
   const int mm0 = (int)((x + q + (uint32_t)(c0 + N - 1)) & 1) == e1 ? c0 - 1 : c0;
-  double up[6];  // upper site of the previous first-parity pair
-  const int npair = (c1 - c0) / 2;
-  for (int k = 0; k <= npair; k++) {
+  // pair k: lower site m1 (parity e1), upper site m2 = m1 + 1 (parity e2)
+  auto pair_at = [&](int k, int &m1, uint32_t &i1, uint32_t &l1, uint32_t &i2, uint32_t &l2)
+      __attribute__((always_inline)) {
     const int mm = mm0 + 2 * k;
-    const int m1 = mm < 0 ? mm + N : (mm >= N ? mm - N : mm);
+    m1 = mm < 0 ? mm + N : (mm >= N ? mm - N : mm);
     const int m2 = m1 + 1 == N ? 0 : m1 + 1;
-    uint32_t i1, l1, i2, l2;
     at(m1, i1, l1);
     at(m2, i2, l2);
+  };
+  // PIPE: pair k+1's loads are issued before pair k's arithmetic, into n1, n2
+  // as stored, and pair k's results are stored in iteration k+1 after those
+  // loads (one in-order counter covers loads and stores: a store issued
+  // between a load and the wait for it would be waited for as well)
+  T n1[6], n2[6], s0[6], s1[6];
+  uint32_t si0 = 0, si1 = 0;
+  auto load_pair = [&](uint32_t i1, uint32_t i2) __attribute__((always_inline)) {
+#pragma unroll
+    for (int L = 4; L < 10; L++) {
+      n1[L - 4] = gplane<T, W>(fb, cur, L, e1).raw(i1);
+      n2[L - 4] = gplane<T, W>(fb, cur, L, e2).raw(i2);
+    }
+  };
+  auto store_pending = [&]() __attribute__((always_inline)) {
+#pragma unroll
+    for (int L = 4; L < 10; L++) {
+      gplane<T, W>(fb, nxt, L, e2).template put_raw<false>(si0, s0[L - 4]);
+      gplane<T, W>(fb, nxt, L, e1).template put_raw<false>(si1, s1[L - 4]);
+    }
+  };
+  const int npair = (c1 - c0) / 2;
+  if (PIPE) {
+    int m1;
+    uint32_t i1, l1, i2, l2;
+    pair_at(0, m1, i1, l1, i2, l2);
+    load_pair(i1, i2);
+  }
+  double up[6];  // upper site of the previous first-parity pair
+  for (int k = 0; k <= npair; k++) {
+    const int mm = mm0 + 2 * k;
+    int m1;
+    uint32_t i1, l1, i2, l2;
+    pair_at(k, m1, i1, l1, i2, l2);
     const bool own1 = mm >= c0 && mm < c1;
     Xoro g1 = xoro_seeded(site_key(k1, l1));
     Xoro g2;
@@ -655,14 +688,30 @@ __device__ __forceinline__ void charge_column(const ReplicaParams &p, int cur, i
       at(m0, i0, l0);
       g2 = xoro_seeded(site_key(k2, l0));
     }
-    // every load of the pair is issued before the first store: the compiler
-    // cannot prove that the stores (other buffer) miss the loaded planes and
-    // would otherwise wait for each component's load after the previous store
     double w1[6], w2[6];
+    if (PIPE) {
 #pragma unroll
-    for (int L = 4; L < 10; L++) {
-      w1[L - 4] = gplane<T, W>(fb, cur, L, e1).ld(i1);
-      w2[L - 4] = gplane<T, W>(fb, cur, L, e2).ld(i2);
+      for (int L = 4; L < 10; L++) {
+        w1[L - 4] = (double)n1[L - 4];
+        w2[L - 4] = (double)n2[L - 4];
+      }
+      if (k < npair) {
+        int mn;
+        uint32_t j1, ln1, j2, ln2;
+        pair_at(k + 1, mn, j1, ln1, j2, ln2);
+        load_pair(j1, j2);
+      }
+      if (k > 1) store_pending();
+    } else {
+      // every load of the pair is issued before the first store: the
+      // compiler cannot prove that the stores (other buffer) miss the loaded
+      // planes and would otherwise wait for each component's load after the
+      // previous store
+#pragma unroll
+      for (int L = 4; L < 10; L++) {
+        w1[L - 4] = gplane<T, W>(fb, cur, L, e1).ld(i1);
+        w2[L - 4] = gplane<T, W>(fb, cur, L, e2).ld(i2);
+      }
     }
 #pragma unroll
     for (int L = 4; L < 10; L++) {
@@ -675,12 +724,22 @@ __device__ __forceinline__ void charge_column(const ReplicaParams &p, int cur, i
       if (k > 0) {
         double o0, o1;
         charge_pair(up[L - 4], a, ck, g2, o0, o1, cnt, dSd, dSu, true);
-        gplane<T, W>(fb, nxt, L, e2).st(i0, o0);
-        gplane<T, W>(fb, nxt, L, e1).st(i1, o1);
+        if (PIPE) {
+          s0[L - 4] = (T)o0;
+          s1[L - 4] = (T)o1;
+        } else {
+          gplane<T, W>(fb, nxt, L, e2).st(i0, o0);
+          gplane<T, W>(fb, nxt, L, e1).st(i1, o1);
+        }
       }
       up[L - 4] = b;
     }
+    if (PIPE && k > 0) {
+      si0 = i0;
+      si1 = i1;
+    }
   }
+  if (PIPE && npair > 0) store_pending();
 }
 
 template <typename T, bool W, bool XW>
@@ -733,8 +792,8 @@ __global__ __launch_bounds__(kBlock) void k_charge_march(const ReplicaParams *__
     } else {
 #pragma unroll 1
       for (int sx = 0; sx < 2; sx++)
-        charge_column<T, W>(p, cur, nxt, dir, e1, e2, 2 * xh + sx, q, c0, c1, k1, k2, cnt, dSd,
-                            dSu);
+        charge_column<T, W, XW>(p, cur, nxt, dir, e1, e2, 2 * xh + sx, q, c0, c1, k1, k2, cnt,
+                                dSd, dSu);
     }
   }
   block_monitor_add(mon + bid, cnt, dSd, dSu);
