@@ -52,22 +52,12 @@ MEASURED_TRAFFIC = {
 }
 
 
-# The library runs the phi z-march for FP64 fields above 1 GiB and FP32
-# fields above 600 MiB (one buffer, all replicas) and the one-thread-per-site
-# phi kernel otherwise.
-LARGE_FIELD_BYTES = {False: 2 ** 30, True: 600 * 2 ** 20}
-
-
-def phi_march(fp32, field_bytes):
-    return field_bytes > LARGE_FIELD_BYTES[fp32]
-
-
-def tables(fp32, field_bytes):
+def tables(fp32, march):
     """(algorithmic bytes per site per launch, kernel name, schedule bytes per
-    site-update) for the precision's kernel schedule; field_bytes (one
-    buffer) decides the phi kernel."""
+    site-update) for the precision's kernel schedule; march: the phi
+    half-sweep runs the z-march."""
     sched = SCHEDULE_BYTES_PER_SITE_UPDATE_FP32 if fp32 else SCHEDULE_BYTES_PER_SITE_UPDATE
-    if phi_march(fp32, field_bytes):
+    if march:
         return ALG_BYTES, KERNEL, sched
     return ALG_BYTES, dict(KERNEL, phi="k_hb_phi"), sched
 
@@ -285,7 +275,7 @@ def run_single(L, N, fp32, steps, warmup, graph=False, tune=True, ideal_tile=Non
         from langevina_amd.autotune import autotune_ideal, autotune_sweeps
         if not ideal_tile:
             tuned = autotune_ideal(ctx, DT)
-        sweeps = autotune_sweeps(ctx, DT, march=phi_march(fp32, N ** 3 * 10 * esz))
+        sweeps = autotune_sweeps(ctx, DT)
     if graph:
         ctx.use_graph(True)
     ctx.run(DT, warmup)
@@ -300,13 +290,15 @@ def run_single(L, N, fp32, steps, warmup, graph=False, tune=True, ideal_tile=Non
     ctx.timing_enable(False)
     sites = N ** 3
     value = sites * steps / el
-    alg, kern, sched = tables(fp32, sites * 10 * esz)
+    march = ctx.phi_kernel() == L.PHI_MARCH
+    alg, kern, sched = tables(fp32, march)
     out = {
         "value": value, "ms_per_step": el * 1e3 / steps, "steps": steps, "warmup": warmup,
         "dtype": "f32-storage/f64-arith" if fp32 else "f64",
         "workload": f"ModelG {N}^3 PV2HBSplit23 {'FP32 fields' if fp32 else 'FP64'}",
         "ideal_tile": list(tuned[N][:2]) if tuned else "default (32, 16)",
         "sweep_tiles": list(sweeps[N]) if sweeps else "default",
+        "phi_kernel": kern["phi"],
         "step_roofline_frac": value * ALG_BYTES_PER_SITE_UPDATE * esz / 8 / HBM_PEAK,
         "step_schedule_bytes_frac": value * sched * esz / 8 / HBM_PEAK,
         "roofline": roofline_of(tm, alg, kern, sites, esz, "fp32" if fp32 else "fp64", N),
@@ -343,7 +335,7 @@ def run_batched(L, fp32, steps, warmup, tune=True, graph=True, graph_steps=10):
     tuned = autotune_ideal(ctx, DT) if tune else {}
     sweeps = {}
     if tune:
-        sweeps = autotune_sweeps(ctx, DT, march=phi_march(fp32, sites * 10 * esz))
+        sweeps = autotune_sweeps(ctx, DT)
     ctx.timing_enable(True)  # kernel breakdown from a short direct-launch pass
     ctx.run(DT, 5)
     tm = ctx.timing()
@@ -356,6 +348,7 @@ def run_batched(L, fp32, steps, warmup, tune=True, graph=True, graph_steps=10):
     ctx.synchronize()
     el = time.perf_counter() - t0
     value = sites * steps / el
+    march = ctx.phi_kernel() == L.PHI_MARCH
     ctx.close()
     seq_el = 0.0
     for p in reps:
@@ -374,7 +367,7 @@ def run_batched(L, fp32, steps, warmup, tune=True, graph=True, graph_steps=10):
         seq_el += time.perf_counter() - t0
         c1.close()
     seq_value = sites * steps / seq_el
-    alg, kern, _ = tables(fp32, sites * 10 * esz)
+    alg, kern, _ = tables(fp32, march)
     return {
         "value": value, "ms_per_step": el * 1e3 / steps, "steps": steps, "warmup": warmup,
         "dtype": "f32-storage/f64-arith" if fp32 else "f64",
@@ -382,6 +375,7 @@ def run_batched(L, fp32, steps, warmup, tune=True, graph=True, graph_steps=10):
                     "per step",
         "sites": sites, "ideal_tiles": {str(k): list(v[:2]) for k, v in tuned.items()},
         "sweep_tiles": {str(k): list(v) for k, v in sweeps.items()},
+        "phi_kernel": kern["phi"],
         "one_at_a_time": {"value": seq_value, "ms_per_step": seq_el * 1e3 / steps},
         "batched_over_one_at_a_time": value / seq_value,
         "step_roofline_frac": value * ALG_BYTES_PER_SITE_UPDATE * esz / 8 / HBM_PEAK,
@@ -522,7 +516,8 @@ def main():
         "data": "synthetic (phi=0, Gaussian charges, zero mode removed; seed 42)",
         "config": {"workload": h["workload"], "N": N, "dt": DT, "params": PARAMS,
                    "parallelism": "1 GPU", "graph": bool(args.graph),
-                   "ideal_tile": h["ideal_tile"], "sweep_tiles": h["sweep_tiles"]},
+                   "ideal_tile": h["ideal_tile"], "sweep_tiles": h["sweep_tiles"],
+                   "phi_kernel": h["phi_kernel"]},
         "step_roofline_frac": h["step_roofline_frac"],
         "step_schedule_bytes_frac": h["step_schedule_bytes_frac"],
         "roofline": h["roofline"], "kernels": h["kernels"], "measurement": h["measurement"],

@@ -184,6 +184,16 @@ int lgv_set_ideal_tile(lgv_ctx *ctx, int replica, int txh_cap, int tz);
  * them.  Monitor counts are kept across the change. */
 int lgv_set_sweep_tiles(lgv_ctx *ctx, int replica, int hb_cap, int hb_tz, int chg_cap,
                         int chg_chunk);
+/* The kernel of the phi half-sweeps (EulerLangevinHB::step,
+ * ModelG/Stepper.cxx:287-389): LGV_PHI_SITE one thread per site,
+ * LGV_PHI_MARCH the z-march with the other parity's plane in LDS,
+ * LGV_PHI_AUTO the size rule (the march for FP64 fields above 1 GiB and FP32
+ * fields above 600 MiB).  Results do not depend on it; the autotuner picks
+ * it.  Monitor counts are kept across the change.  lgv_get_phi_kernel
+ * reports the kernel in use (LGV_PHI_SITE or LGV_PHI_MARCH). */
+enum { LGV_PHI_AUTO = 0, LGV_PHI_SITE = 1, LGV_PHI_MARCH = 2 };
+int lgv_set_phi_kernel(lgv_ctx *ctx, int kernel);
+int lgv_get_phi_kernel(lgv_ctx *ctx, int *kernel);
 
 int lgv_synchronize(lgv_ctx *ctx);
 /* Device address of a replica's field storage (10 component planes, each

@@ -9,6 +9,7 @@ are the same whichever block computes it), only speed.
 import ctypes as C
 import json
 import os
+import time
 
 from .lgv import _check, lib
 
@@ -57,12 +58,22 @@ WARMUP = 3
 
 
 def _class_ms(ctx, dt, steps, cls):
+    """Mean launch time of one kernel class (HIP events), or (cls None) the
+    mean step time of an untimed run on the host clock, with the sweeps in
+    the order (sequential or concurrent) the library runs them."""
     ctx.step(dt)  # the first launches after a retile are not timed
+    if cls is None:
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.run(dt, steps)
+        ctx.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps
     ctx.timing_enable(True)
     for _ in range(steps):
         ctx.step(dt)
-    ms, n = ctx.timing()[cls]
+    tm = ctx.timing()
     ctx.timing_enable(False)
+    ms, n = tm[cls]
     return ms / max(n, 1)
 
 
@@ -88,12 +99,15 @@ def _search(ctx, dt, steps, cls, apply, candidates):
     return best, best_ms[best]
 
 
-def autotune_sweeps(ctx, dt, march, steps=3, verbose=False):
-    """Per size class, as autotune_ideal: first the charge march (tile
-    width x chunk), then, when the phi half-sweep runs the z-march
-    (march=True: FP64 fields above 1 GiB, FP32 above 600 MiB), its tile
-    width x z-chunk.  Returns {N: (hb_cap, hb_tz, chg_cap, chg_chunk)}
-    (0 = default)."""
+def autotune_sweeps(ctx, dt, march=None, steps=3, verbose=False):
+    """march None: first the phi half-sweep kernel, the per-site kernel or
+    the z-march, timed over whole steps (the march's lower traffic also
+    speeds up the kernels that follow it), the size rule's choice being the
+    incumbent; march True / False: keep the kernel in use and tune (or not)
+    the march's tiles.  Then per size class, as autotune_ideal: the charge
+    march (tile width x chunk) and, when the phi half-sweep runs the z-march,
+    its tile width x z-chunk.  Returns {N: (hb_cap, hb_tz, chg_cap,
+    chg_chunk)} (0 = default); ctx.phi_kernel() reports the kernel."""
     sizes = sorted(set(ctx.N))
     chosen = {N: [0, 0, 0, 0] for N in sizes}
 
@@ -107,6 +121,16 @@ def autotune_sweeps(ctx, dt, march, steps=3, verbose=False):
 
     for _ in range(WARMUP):
         ctx.step(dt)
+    if march is None:
+        from .lgv import PHI_AUTO, PHI_MARCH, PHI_SITE
+        ctx.set_phi_kernel(PHI_AUTO)
+        inc = ctx.phi_kernel()
+        kernel, ms = _search(ctx, dt, 4 * steps, None, ctx.set_phi_kernel,
+                             [inc, PHI_SITE if inc == PHI_MARCH else PHI_MARCH])
+        march = kernel == PHI_MARCH
+        if verbose:
+            print(f"autotune phi kernel: {'march' if march else 'site'} {ms:.3f} ms/step",
+                  flush=True)
     for N in sizes:
         cands, seen = [(0, 0)], {charge_eff(N, 0, 0)}
         for cap in CHARGE_CAPS:

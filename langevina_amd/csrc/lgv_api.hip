@@ -100,6 +100,7 @@ struct lgv_ctx {
   bool wide = false;    // some plane is >= 4 GiB: kernels use 64-bit offsets
   bool nt_stores = false;  // one buffer's fields exceed kStreamBytes
   bool phi_march = false;  // the phi half-sweep runs the z-march kernel
+  bool phi_march_auto = false;  // the size rule's choice (LGV_PHI_AUTO)
   MonSlot *mon_phi = nullptr, *mon_charge = nullptr, *mon_ideal = nullptr;
   // monitor counts folded out of the per-block slots when a retile
   // reallocated them: [replica][phi, charge]
@@ -464,8 +465,6 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
   HIPCHECK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
   HIPCHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
   HIPCHECK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
-  // test hook: LGV_OVERLAP=0 runs the charge sweep after the phi sweeps
-  if (const char *ov = getenv("LGV_OVERLAP")) c->overlap = atoi(ov) != 0;
   c->fp32 = cfg->precision == LGV_FP32;
   c->s0 = s0;
   c->s1 = s1;
@@ -525,6 +524,14 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
   // test hook: LGV_NT_STORES=0/1 forces the store mode on any size
   if (const char *nt = getenv("LGV_NT_STORES")) c->nt_stores = atoi(nt) != 0;
   c->phi_march = c->fp32 ? buffer_bytes > kMarchBytesFp32 : c->nt_stores;
+  c->phi_march_auto = c->phi_march;
+  // one lattice whose launches each fill the GPU many times over gains
+  // nothing from running the charge sweep beside the phi sweeps (256^3:
+  // 1 % slower); smaller and batched lattices do (128^3 FP64 2.53e9 ->
+  // 2.67e9 site-updates/s, the 96-replica FP32 batch +2 %)
+  c->overlap = !(cfg->nreplicas == 1 && buffer_bytes > kMarchBytesFp32);
+  // test hook: LGV_OVERLAP=0/1 forces the sequential / concurrent order
+  if (const char *ov = getenv("LGV_OVERLAP")) c->overlap = atoi(ov) != 0;
   // test hook: LGV_PHI_KERNEL=site/march forces the phi half-sweep kernel
   if (const char *pk = getenv("LGV_PHI_KERNEL")) {
     const std::string k(pk);
@@ -668,6 +675,29 @@ static int drop_graphs(lgv_ctx *c) {
     if (*g) HIPCHECK(hipGraphExecDestroy(*g));
     *g = nullptr;
   }
+  return LGV_OK;
+}
+
+// The per-block monitor slots follow the plans: before a replanning,
+// fold_monitors adds them into the per-replica carry; realloc_monitors
+// then allocates zeroed slots for the plans in use.
+static int fold_monitors(lgv_ctx *c) {
+  lgv_monitor m[3];
+  for (int i = 0; i < c->nrep; i++) {
+    int rc = lgv_monitor_get(c, i, m);
+    if (rc) return rc;
+    c->mon_carry[i] = {m[0], m[1]};
+  }
+  return LGV_OK;
+}
+static int realloc_monitors(lgv_ctx *c) {
+  HIPCHECK(hipFree(c->mon_phi));
+  HIPCHECK(hipFree(c->mon_charge));
+  c->mon_phi = c->mon_charge = nullptr;
+  HIPCHECK(hipMalloc(&c->mon_phi, hb_plan(c).nblocks * sizeof(MonSlot)));
+  HIPCHECK(hipMemset(c->mon_phi, 0, hb_plan(c).nblocks * sizeof(MonSlot)));
+  HIPCHECK(hipMalloc(&c->mon_charge, c->plan_charge.nblocks * sizeof(MonSlot)));
+  HIPCHECK(hipMemset(c->mon_charge, 0, c->plan_charge.nblocks * sizeof(MonSlot)));
   return LGV_OK;
 }
 
@@ -1047,16 +1077,7 @@ int lgv_set_sweep_tiles(lgv_ctx *c, int r, int hb_cap, int hb_tz, int chg_cap,
                 chg_chunk);
   HIPCHECK(hipSetDevice(c->device));
   HIPCHECK(hipStreamSynchronize(c->stream));
-  // the per-block monitor slots follow the plans: fold them into the
-  // per-replica carry before they are reallocated
-  {
-    lgv_monitor m[3];
-    for (int i = 0; i < c->nrep; i++) {
-      int rc = lgv_monitor_get(c, i, m);
-      if (rc) return rc;
-      c->mon_carry[i] = {m[0], m[1]};
-    }
-  }
+  if (int rm = fold_monitors(c)) return rm;
   std::vector<ReplicaParams> old = c->hp;
   for (int i = 0; i < c->nrep; i++) {
     if (r >= 0 && i != r) continue;
@@ -1072,18 +1093,33 @@ int lgv_set_sweep_tiles(lgv_ctx *c, int r, int hb_cap, int hb_tz, int chg_cap,
     build_hb_plan(c);
     build_charge_plan(c);
   }
-  HIPCHECK(hipFree(c->mon_phi));
-  HIPCHECK(hipFree(c->mon_charge));
-  c->mon_phi = c->mon_charge = nullptr;
-  HIPCHECK(hipMalloc(&c->mon_phi, hb_plan(c).nblocks * sizeof(MonSlot)));
-  HIPCHECK(hipMemset(c->mon_phi, 0, hb_plan(c).nblocks * sizeof(MonSlot)));
-  HIPCHECK(hipMalloc(&c->mon_charge, c->plan_charge.nblocks * sizeof(MonSlot)));
-  HIPCHECK(hipMemset(c->mon_charge, 0, c->plan_charge.nblocks * sizeof(MonSlot)));
+  if (int rm = realloc_monitors(c)) return rm;
   HIPCHECK(hipMemcpy(c->dP, c->hp.data(), c->nrep * sizeof(ReplicaParams),
                      hipMemcpyHostToDevice));
   HIPCHECK(hipDeviceSynchronize());
   if (int rg = drop_graphs(c)) return rg;
   return rc;
+}
+
+int lgv_set_phi_kernel(lgv_ctx *c, int kernel) {
+  if (!c) return fail(LGV_ERR_ARG, "null context");
+  if (kernel != LGV_PHI_AUTO && kernel != LGV_PHI_SITE && kernel != LGV_PHI_MARCH)
+    return fail(LGV_ERR_ARG, "phi kernel %d: expected LGV_PHI_AUTO, _SITE or _MARCH", kernel);
+  const bool march = kernel == LGV_PHI_AUTO ? c->phi_march_auto : kernel == LGV_PHI_MARCH;
+  if (march == c->phi_march) return LGV_OK;
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  if (int rm = fold_monitors(c)) return rm;
+  c->phi_march = march;
+  if (int rm = realloc_monitors(c)) return rm;
+  HIPCHECK(hipDeviceSynchronize());
+  return drop_graphs(c);
+}
+
+int lgv_get_phi_kernel(lgv_ctx *c, int *kernel) {
+  if (!c || !kernel) return fail(LGV_ERR_ARG, "null argument");
+  *kernel = c->phi_march ? LGV_PHI_MARCH : LGV_PHI_SITE;
+  return LGV_OK;
 }
 
 int lgv_monitor_reset(lgv_ctx *c) {

@@ -61,8 +61,11 @@ EXPORTS = [
     "lgv_set_mass", "lgv_monitor_reset", "lgv_set_ideal_tile", "lgv_set_sweep_tiles",
     "lgv_em_last_error", "lgv_em_create", "lgv_em_set_stream", "lgv_em_forward",
     "lgv_em_backward", "lgv_em_destroy", "lgv_em_series", "lgv_em_step",
-    "lgv_em_step_noalloc",
+    "lgv_em_step_noalloc", "lgv_set_phi_kernel", "lgv_get_phi_kernel",
 ]
+
+# lgv_set_phi_kernel values
+PHI_AUTO, PHI_SITE, PHI_MARCH = 0, 1, 2
 
 
 class Measurement(C.Structure):
@@ -297,6 +300,16 @@ class Context:
 
     def synchronize(self):
         _check(lib().lgv_synchronize(self.h))
+
+    def set_phi_kernel(self, kernel):
+        """PHI_AUTO (size rule), PHI_SITE or PHI_MARCH for the phi half-sweeps."""
+        _check(lib().lgv_set_phi_kernel(self.h, int(kernel)))
+
+    def phi_kernel(self):
+        """The phi half-sweep kernel in use: PHI_SITE or PHI_MARCH."""
+        k = C.c_int()
+        _check(lib().lgv_get_phi_kernel(self.h, C.byref(k)))
+        return k.value
 
     def timing_enable(self, enable=True):
         _check(lib().lgv_timing_enable(self.h, int(enable)))

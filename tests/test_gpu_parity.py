@@ -473,6 +473,39 @@ def test_sweep_tiles_do_not_change_results(L, monkeypatch, N):
                     assert mon[kind][k] == ref[1][kind][k], (tiles, kind, k)
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_phi_kernel_switch_keeps_results(L, prec):
+    """lgv_set_phi_kernel: switching between the per-site kernel and the
+    z-march in the middle of a run gives the bits and acceptance counts of a
+    run on either kernel alone; LGV_PHI_AUTO restores the size rule's
+    choice (the per-site kernel at 24^3); other values are rejected."""
+    log("phi kernel switch", prec)
+    N = 24
+    u = np.random.default_rng(7).normal(size=(N, N, N, 10)) * 0.6
+    runs = {}
+    for name, kernels in (("site", (L.PHI_SITE, L.PHI_SITE)),
+                          ("march", (L.PHI_MARCH, L.PHI_MARCH)),
+                          ("switch", (L.PHI_MARCH, L.PHI_SITE))):
+        ctx = L.Context(L.model_params(N, seed=5, **PAR), precision=prec)
+        ctx.set_field(u)
+        for k in kernels:
+            ctx.set_phi_kernel(k)
+            assert ctx.phi_kernel() == k
+            ctx.run(0.04, 2)
+        runs[name] = (ctx.get_field(), ctx.monitors())
+        if name == "switch":
+            ctx.set_phi_kernel(L.PHI_AUTO)
+            assert ctx.phi_kernel() == L.PHI_SITE
+            with pytest.raises(L.LgvError, match="phi kernel"):
+                ctx.set_phi_kernel(3)
+        ctx.close()
+    for name in ("march", "switch"):
+        np.testing.assert_array_equal(runs[name][0], runs["site"][0], err_msg=name)
+        for kind in ("phi", "charge"):
+            for k in ("down_counts", "up_yes", "up_no"):
+                assert runs[name][1][kind][k] == runs["site"][1][kind][k], (name, kind, k)
+
+
 def test_sweep_tiles_are_checked(L):
     from langevina_amd.autotune import set_sweep_tiles
     ctx = L.Context(L.model_params(16, seed=1, **PAR))
