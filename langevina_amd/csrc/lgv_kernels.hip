@@ -977,20 +977,21 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
         // held to 1e-12 like the rotations: contraction into FMA allowed
 #pragma clang fp contract(fast)
         const int lx = 2 * tx + ((pp + y + z) & 1) + 1, ly = ty + 1;
-        double cp[4], xp[4], xm[4], yp[4], ym[4];
+        // The kick of Stepper.cxx:112-163, summed per direction there, is
+        // linear in the neighbours: with S the sum of the six neighbours'
+        // rotated phi, the axial kick is axx (cp0 S_l - cp_l S0) - H cp_l
+        // and the vector kick axx (cp_s1 S_s2 - cp_s2 S_s1), (s1, s2) the
+        // cyclic successors of s (every direction shares axx).
+        const double *zp = nex[pp ^ 1];
+        double cp[4], S[4], A[3], V[3];
 #pragma unroll
         for (int c = 0; c < 4; c++) {
           cp[c] = cen[pp][c];
-          xp[c] = lds[c][ly * LW + lx + 1];
-          xm[c] = lds[c][ly * LW + lx - 1];
-          yp[c] = lds[c][(ly + 1) * LW + lx];
-          ym[c] = lds[c][(ly - 1) * LW + lx];
+          S[c] = (lds[c][ly * LW + lx + 1] + lds[c][ly * LW + lx - 1]) +
+                 (lds[c][(ly + 1) * LW + lx] + lds[c][(ly - 1) * LW + lx]) +
+                 (zp[c] + lprv[pp ^ 1][c][t]);
         }
         const int par = pp;
-        const double *zp = nex[pp ^ 1];
-        double zm[4], A[3], V[3];
-#pragma unroll
-        for (int c = 0; c < 4; c++) zm[c] = lprv[pp ^ 1][c][t];
         // the charges of plane z, unchanged by the first rotation
 #pragma unroll
         for (int l = 0; l < 3; l++) {
@@ -1003,25 +1004,12 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
           }
         }
 #pragma unroll
-        for (int l = 0; l < 3; l++) {
-          const double advxx = (-xp[0] * cp[l + 1] + cp[0] * xp[l + 1] + cp[0] * xm[l + 1] -
-                                xm[0] * cp[l + 1]) * axx;
-          const double advyy = (-yp[0] * cp[l + 1] + cp[0] * yp[l + 1] + cp[0] * ym[l + 1] -
-                                ym[0] * cp[l + 1]) * axx;
-          const double advzz = (-zp[0] * cp[l + 1] + cp[0] * zp[l + 1] + cp[0] * zm[l + 1] -
-                                zm[0] * cp[l + 1]) * axx;
-          A[l] += dt * (advxx + advyy + advzz - H0 * cp[l + 1]);
-        }
+        for (int l = 0; l < 3; l++)
+          A[l] += dt * (axx * (cp[0] * S[l + 1] - cp[l + 1] * S[0]) - H0 * cp[l + 1]);
 #pragma unroll
         for (int s3 = 0; s3 < 3; s3++) {
           const int s1 = (s3 + 1) % 3, s2 = (s3 + 2) % 3;
-          const double advxx = (xp[1 + s2] * cp[1 + s1] - cp[1 + s2] * xm[1 + s1] -
-                                xp[1 + s1] * cp[1 + s2] + cp[1 + s1] * xm[1 + s2]) * axx;
-          const double advyy = (yp[1 + s2] * cp[1 + s1] - cp[1 + s2] * ym[1 + s1] -
-                                yp[1 + s1] * cp[1 + s2] + cp[1 + s1] * ym[1 + s2]) * axx;
-          const double advzz = (zp[1 + s2] * cp[1 + s1] - cp[1 + s2] * zm[1 + s1] -
-                                zp[1 + s1] * cp[1 + s2] + cp[1 + s1] * zm[1 + s2]) * axx;
-          V[s3] += dt * (advxx + advyy + advzz);
+          V[s3] += dt * (axx * (cp[1 + s1] * S[1 + s2] - cp[1 + s2] * S[1 + s1]));
         }
         rotate_site_fast(cp, A, V, h);
 #pragma unroll
