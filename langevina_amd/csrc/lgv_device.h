@@ -193,7 +193,9 @@ LGV_HD int site_x(int xh, int y, int z, int p) { return 2 * xh + ((p + y + z) & 
 // With k = tau/chi the two generators are tmp1 = -0.5(mu_V - mu_A) =
 // h (V - A) and tmp2 = 0.5(mu_V + mu_A) = -h (V + A), h = k/2.  expmul needs
 // cos|v| and sin|v|/|v|; for |v| < 1/2 both are evaluated as Taylor series
-// in y = |v|^2 (truncation below 1e-18), otherwise with sqrt/sincos.  Agrees
+// in y = |v|^2 (truncation below 1e-18; degree 4 below y = 2^-10, where
+// the charges of a thermal lattice put almost every site), otherwise with
+// sqrt/sincos.  Agrees
 // with the reference's expression order to a few ulps (the reference's own
 // libm calls are not bit-reproducible on the device anyway).
 // The ideal step's arithmetic is held to 1e-12, not to bits, so these
@@ -201,7 +203,11 @@ LGV_HD int site_x(int xh, int y, int z, int p) { return 2 * xh + ((p + y + z) & 
 // rest of the library is built with -ffp-contract=off).
 LGV_HD void cos_sinc(double y, double &c, double &s) {
 #pragma clang fp contract(fast)
-  if (y < 0.25) {
+  if (y < 0x1p-10) {
+    // the first omitted terms, y^5/10! and y^5/11!, are below 3e-22
+    c = 1.0 + y * (-1.0 / 2 + y * (1.0 / 24 + y * (-1.0 / 720 + y * (1.0 / 40320))));
+    s = 1.0 + y * (-1.0 / 6 + y * (1.0 / 120 + y * (-1.0 / 5040 + y * (1.0 / 362880))));
+  } else if (y < 0.25) {
     c = 1.0 + y * (-1.0 / 2 + y * (1.0 / 24 + y * (-1.0 / 720 + y * (1.0 / 40320 +
         y * (-1.0 / 3628800 + y * (1.0 / 479001600 + y * (-1.0 / 87178291200.0)))))));
     s = 1.0 + y * (-1.0 / 6 + y * (1.0 / 120 + y * (-1.0 / 5040 + y * (1.0 / 362880 +

@@ -900,11 +900,17 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
   auto rotate_own = [&](int z, double out[2][4], T ch[2][6]) __attribute__((always_inline)) {
     if (!own) return;
     const uint32_t i = ((uint32_t)z * N + y) * NH + xh;
+    // both sites' loads before the first rotation: one wait for the pair
+    T raw[2][10];
+#pragma unroll
+    for (int pp = 0; pp < 2; pp++)
+#pragma unroll
+      for (int c = 0; c < 10; c++) raw[pp][c] = gplane<T, W>(p, cur, c, pp).raw(i);
 #pragma unroll
     for (int pp = 0; pp < 2; pp++) {
       double f[10];
 #pragma unroll
-      for (int c = 0; c < 10; c++) f[c] = gplane<T, W>(p, cur, c, pp).ld(i);
+      for (int c = 0; c < 10; c++) f[c] = (double)raw[pp][c];
       rotate_site_fast(f, f + 4, f + 7, h);
 #pragma unroll
       for (int c = 0; c < 4; c++) out[pp][c] = f[c];
