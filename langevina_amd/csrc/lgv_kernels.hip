@@ -823,7 +823,8 @@ __device__ __forceinline__ void load_site(const ReplicaParams &p, int cur, int x
 }
 
 template <typename T, bool W, bool NTS>
-__global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__restrict__ P,
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1))) void k_ideal_fused(const ReplicaParams *__restrict__ P,
                                                         const ReplicaState *__restrict__ S,
                                                         const int *__restrict__ blk_rep,
                                                         const int *__restrict__ blk_first) {
@@ -888,7 +889,7 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
   // FP64 fields also load the raw fields of the thread's first ring site of
   // plane z+1 before the kick of plane z, so that their latency overlaps the
   // kick (free at FP64's 2 waves per SIMD; FP32 would drop from 3 waves to 2)
-  constexpr bool kEarlyRing = sizeof(T) == 8;
+  constexpr bool kEarlyRing = true;
   // The thread's two own sites of a plane are indexed by parity (pp), not by
   // x offset: a parity's plane base is the same for every lane, so the loads
   // and stores use one scalar base and 32-bit offsets.  The site of parity pp
@@ -972,9 +973,14 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
     const int z = z0 + k;
     const int zn = (z + 1) % N;
     rotate_own(zn, nex, chn);
-    double rf[10];
+    T rraw[10];  // the ring site's raw fields of plane z+1, as stored
     const bool ring1 = kEarlyRing && t < nh && k + 1 < tz;
-    if (ring1) load_site<T, W>(p, cur, rgx, rgy, zn, rf);
+    if (ring1) {
+      const uint32_t ri = ((uint32_t)zn * N + rgy) * NH + (rgx >> 1);
+      const int rpar = (rgx + rgy + zn) & 1;
+#pragma unroll
+      for (int c = 0; c < 10; c++) rraw[c] = gplane<T, W>(p, cur, c, rpar).raw(ri);
+    }
     __syncthreads();
     if (own) {
       const uint32_t i = ((uint32_t)z * N + y) * NH + xh;
@@ -1028,6 +1034,9 @@ __global__ __launch_bounds__(kBlock) void k_ideal_fused(const ReplicaParams *__r
     }
     __syncthreads();
     if (ring1) {
+      double rf[10];
+#pragma unroll
+      for (int c = 0; c < 10; c++) rf[c] = (double)rraw[c];
       rotate_site_fast(rf, rf + 4, rf + 7, h);
 #pragma unroll
       for (int c = 0; c < 4; c++) lds[c][rpos] = rf[c];
