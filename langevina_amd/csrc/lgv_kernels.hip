@@ -822,12 +822,11 @@ __device__ __forceinline__ void load_site(const ReplicaParams &p, int cur, int x
   for (int c = 0; c < 10; c++) f[c] = gplane<T, W>(p, cur, c, par).ld(i);
 }
 
+// FP32 fields: at least 3 waves per SIMD (168 registers)
 template <typename T, bool W, bool NTS>
-__global__ __launch_bounds__(kBlock)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1))) void k_ideal_fused(const ReplicaParams *__restrict__ P,
-                                                        const ReplicaState *__restrict__ S,
-                                                        const int *__restrict__ blk_rep,
-                                                        const int *__restrict__ blk_first) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1)))
+void k_ideal_fused(const ReplicaParams *__restrict__ P, const ReplicaState *__restrict__ S,
+                   const int *__restrict__ blk_rep, const int *__restrict__ blk_first) {
   __shared__ double lds[4][ideal_lds(sizeof(T) == 4)];
   // rotated phi of plane z-1 at the thread's own sites: thread-private slots
   __shared__ double lprv[2][4][kBlock];
@@ -886,9 +885,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1))) void k_ideal_fused(
   // read 1.92x its algorithmic bytes (the re-reads miss the 4 MB L2).
   constexpr bool kCarry = sizeof(T) == 8;
   __shared__ T chq[kCarry ? 1 : 2][kCarry ? 1 : 6][kCarry ? 1 : kBlock];
-  // FP64 fields also load the raw fields of the thread's first ring site of
-  // plane z+1 before the kick of plane z, so that their latency overlaps the
-  // kick (free at FP64's 2 waves per SIMD; FP32 would drop from 3 waves to 2)
+  // The raw fields of the thread's first ring site of plane z+1 are loaded
+  // before the kick of plane z, so that their latency overlaps the kick
+  // (FP32: ten registers more, which the waves-per-SIMD bound of 3 holds at
+  // 168 with three spilled; 256^3 FP32 347 -> 329 us, 2 waves 360 us)
   constexpr bool kEarlyRing = true;
   // The thread's two own sites of a plane are indexed by parity (pp), not by
   // x offset: a parity's plane base is the same for every lane, so the loads
