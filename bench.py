@@ -253,8 +253,17 @@ def roofline_of(tm, alg, kern, sites, esz, precision, N=None):
             "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": per_launch_s * 1e6}
 
 
+def pin_phi_kernel(L, ctx, phi_kernel):
+    """Sets the phi kernel ('site' / 'march'); returns the autotuner's march
+    argument (None: the tuner chooses)."""
+    if phi_kernel is None:
+        return None
+    ctx.set_phi_kernel(L.PHI_MARCH if phi_kernel == "march" else L.PHI_SITE)
+    return phi_kernel == "march"
+
+
 def run_single(L, N, fp32, steps, warmup, graph=False, tune=True, ideal_tile=None,
-               measure=False):
+               measure=False, phi_kernel=None):
     """One N^3 lattice (phi = 0, Gaussian charges): warmup steps, then steps
     timed with a host clock around a synchronised run and with HIP events per
     kernel class.  Returns the measured figures."""
@@ -264,6 +273,7 @@ def run_single(L, N, fp32, steps, warmup, graph=False, tune=True, ideal_tile=Non
     # synthetic start: phi = 0 (ModelA::initialize), Gaussian charges with the
     # zero mode removed (ModelA::initialize_gaussian_charges)
     ctx.init_gaussian_charges()
+    march = pin_phi_kernel(L, ctx, phi_kernel)
     tuned = {}
     if ideal_tile:
         from langevina_amd.autotune import set_ideal_tile
@@ -275,7 +285,7 @@ def run_single(L, N, fp32, steps, warmup, graph=False, tune=True, ideal_tile=Non
         from langevina_amd.autotune import autotune_ideal, autotune_sweeps
         if not ideal_tile:
             tuned = autotune_ideal(ctx, DT)
-        sweeps = autotune_sweeps(ctx, DT)
+        sweeps = autotune_sweeps(ctx, DT, march=march)
     if graph:
         ctx.use_graph(True)
     ctx.run(DT, warmup)
@@ -318,7 +328,8 @@ def run_single(L, N, fp32, steps, warmup, graph=False, tune=True, ideal_tile=Non
     return out
 
 
-def run_batched(L, fp32, steps, warmup, tune=True, graph=True, graph_steps=10):
+def run_batched(L, fp32, steps, warmup, tune=True, graph=True, graph_steps=10,
+                phi_kernel=None):
     """The 96-replica ensemble in one context, one HIP graph per step, against
     the same replicas one context (and one graph) at a time; each graph holds
     graph_steps steps; graph=False launches the kernels on the stream instead
@@ -332,10 +343,11 @@ def run_batched(L, fp32, steps, warmup, tune=True, graph=True, graph_steps=10):
     ctx = L.Context(reps, precision=prec)
     for r in range(len(reps)):
         ctx.init_gaussian_charges(r)
+    march = pin_phi_kernel(L, ctx, phi_kernel)
     tuned = autotune_ideal(ctx, DT) if tune else {}
     sweeps = {}
     if tune:
-        sweeps = autotune_sweeps(ctx, DT)
+        sweeps = autotune_sweeps(ctx, DT, march=march)
     ctx.timing_enable(True)  # kernel breakdown from a short direct-launch pass
     ctx.run(DT, 5)
     tm = ctx.timing()
@@ -388,7 +400,7 @@ def bench_batched(args):
     import langevina_amd as L
     b = run_batched(L, args.precision == "fp32", args.steps, args.warmup,
                     tune=not args.no_autotune, graph=not args.no_graph,
-                    graph_steps=args.graph_steps)
+                    graph_steps=args.graph_steps, phi_kernel=args.phi_kernel)
     out = {"metric": METRIC, "value": b.pop("value"), "unit": "site-updates/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": b.pop("ms_per_step"),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -477,6 +489,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--ideal-tile", type=int, nargs=2, metavar=("CAP", "TZ"),
                     help="fix the ideal-kernel tile instead of autotuning it")
+    ap.add_argument("--phi-kernel", choices=["site", "march"],
+                    help="fix the phi half-sweep kernel instead of letting the tuner pick it")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the copy kernel used to scale HBM counters")
@@ -506,7 +520,8 @@ def main():
     N = args.size
     fp32 = args.precision == "fp32"
     h = run_single(L, N, fp32, args.steps, args.warmup, graph=args.graph,
-                   tune=not args.no_autotune, ideal_tile=args.ideal_tile, measure=True)
+                   tune=not args.no_autotune, ideal_tile=args.ideal_tile, measure=True,
+                   phi_kernel=args.phi_kernel)
     progress(f"{h['workload']}: {h['value']:.3e} site-updates/s")
     out = {
         "metric": METRIC, "value": h["value"], "unit": "site-updates/s", "n_gpus": 1,

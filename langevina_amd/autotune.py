@@ -125,7 +125,7 @@ def autotune_sweeps(ctx, dt, march=None, steps=3, verbose=False):
         from .lgv import PHI_AUTO, PHI_MARCH, PHI_SITE
         ctx.set_phi_kernel(PHI_AUTO)
         inc = ctx.phi_kernel()
-        kernel, ms = _search(ctx, dt, 4 * steps, None, ctx.set_phi_kernel,
+        kernel, ms = _search(ctx, dt, 8 * steps, None, ctx.set_phi_kernel,
                              [inc, PHI_SITE if inc == PHI_MARCH else PHI_MARCH])
         march = kernel == PHI_MARCH
         if verbose:

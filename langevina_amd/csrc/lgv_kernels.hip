@@ -542,25 +542,29 @@ __device__ __forceinline__ void charge_line(const ReplicaParams &p, int cur, int
   }
 }
 
-// The x direction with a whole row in one wavefront (NH = 64 H): lane l
-// owns the second-parity sites A'_h of half-columns h = l + 64 j, j < H.
-// First sub-sweep: the lane updates the pair (L_h, A'_h), L_h = A'_h - e_x.
-// The updated L_{h+1} is B'_h = A'_h + e_x; it comes from the lane that owns
-// h + 1 (lane l + 1, or for lane 63 lane 0 at j + 1, the row wrapping
-// periodically).  Second sub-sweep: the lane updates (A'_h, B'_h) and
-// writes both.  Every pair is computed once and counted once, from the same
-// counter streams as charge_line.  Index logic checked on the CPU against
-// the plain two sub-sweeps (exact integer stand-in, N = 128, 256).
+// The x direction with a whole row in one wavefront or in one aligned
+// segment of seg lanes of it (NH = seg H; seg = NH <= 64 a power of two for
+// H = 1, seg = 64 for H = 2): lane l of the segment owns the second-parity
+// sites A'_h of half-columns h = l + 64 j, j < H.  First sub-sweep: the lane
+// updates the pair (L_h, A'_h), L_h = A'_h - e_x.  The updated L_{h+1} is
+// B'_h = A'_h + e_x; it comes from the lane that owns h + 1 (lane l + 1, or
+// for the segment's last lane its lane 0, at j + 1 for H = 2, the row
+// wrapping periodically).  Second sub-sweep: the lane updates (A'_h, B'_h)
+// and writes both.  Every pair is computed once and counted once, from the
+// same counter streams as charge_line.  Index logic checked on the CPU
+// against the plain two sub-sweeps (exact integer stand-in, N = 4 ... 256).
 template <typename T, bool W, int H>
 __device__ __forceinline__ void charge_row_wave(const ReplicaParams &p, int cur, int nxt, int e1,
-                                                int e2, uint32_t y, uint32_t z, int lane,
+                                                int e2, uint32_t y, uint32_t z, int lane, int seg,
                                                 uint64_t k1, uint64_t k2, uint64_t &cnt,
                                                 double &dSd, double &dSu) {
+  static_assert(H == 1 || H == 2, "rows of up to 128 half-columns");
   const uint32_t N = p.N, NH = p.NH;
   const uint32_t row = z * N + y;
   const ChargeK ck = charge_k(p.rms, p.chi);
   const FieldBufs fb = field_bufs<T>(p);
-  // a1: updated A'_h; r: updated L_{h+1} from lane + 1; l0: lane 0's updated L
+  // a1: updated A'_h; r: updated L_{h+1} from lane + 1; l0: lane 0's updated
+  // L (H = 2 only)
   double a1[H][6], r[H][6], l0[H][6];
 #pragma unroll
   for (int j = 0; j < H; j++) {
@@ -582,8 +586,11 @@ __device__ __forceinline__ void charge_row_wave(const ReplicaParams &p, int cur,
       // the first sub-sweep's results as stored (rounded in FP32 storage)
       tl = (double)(T)tl;
       a1[j][c] = (double)(T)ta;
-      r[j][c] = __shfl(tl, (lane + 1) & 63, 64);
-      {  // lane 0's value, broadcast through scalar registers
+      if constexpr (H == 1) {
+        r[j][c] = __shfl(tl, (lane + 1) & (seg - 1), seg);
+      } else {
+        r[j][c] = __shfl(tl, (lane + 1) & 63, 64);
+        // lane 0's value, broadcast through scalar registers
         const long long bits = __double_as_longlong(tl);
         const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)bits);
         const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(bits >> 32));
@@ -599,8 +606,8 @@ __device__ __forceinline__ void charge_row_wave(const ReplicaParams &p, int cur,
     Xoro gA = xoro_seeded(site_key(k2, row * N + x));
 #pragma unroll
     for (int c = 0; c < 6; c++) {
-      // lane 63's B'_h is lane 0's L of the next j (wrapping to j = 0)
-      const double nb = lane == 63 ? l0[(j + 1) % H][c] : r[j][c];
+      // H = 2: lane 63's B'_h is lane 0's L of the next j (wrapping to j = 0)
+      const double nb = (H > 1 && lane == 63) ? l0[(j + 1) % H][c] : r[j][c];
       double a2, b2;
       charge_pair(a1[j][c], nb, ck, gA, a2, b2, cnt, dSd, dSu, true);
       gplane<T, W>(fb, nxt, 4 + c, e2).st(row * NH + h, a2);
@@ -780,12 +787,19 @@ __global__ __launch_bounds__(kBlock) void k_charge_march(const ReplicaParams *__
       const int zs = c0 + bx * len / nbx, ze = c0 + (bx + 1) * len / nbx;
       for (int z = zs; z < ze; z++) {
         if (NH == 64)
-          charge_row_wave<T, W, 1>(p, cur, nxt, e1, e2, q, (uint32_t)z, lane, k1, k2, cnt, dSd,
-                                   dSu);
+          charge_row_wave<T, W, 1>(p, cur, nxt, e1, e2, q, (uint32_t)z, lane, 64, k1, k2, cnt,
+                                   dSd, dSu);
         else
-          charge_row_wave<T, W, 2>(p, cur, nxt, e1, e2, q, (uint32_t)z, lane, k1, k2, cnt, dSd,
-                                   dSu);
+          charge_row_wave<T, W, 2>(p, cur, nxt, e1, e2, q, (uint32_t)z, lane, 64, k1, k2, cnt,
+                                   dSd, dSu);
       }
+    } else if (dir == 0 && TXH == NH && NH <= 64 && (NH & (NH - 1)) == 0) {
+      // whole rows in aligned segments of NH lanes: with TXH = NH thread t
+      // takes half-column t % NH of the tile's row t / NH, and NH divides
+      // the wavefront, so a row never straddles two waves
+      for (int z = c0; z < c1; z++)
+        charge_row_wave<T, W, 1>(p, cur, nxt, e1, e2, q, (uint32_t)z, (int)xh, NH, k1, k2, cnt,
+                                 dSd, dSu);
     } else if (dir == 0) {
       for (int z = c0; z < c1; z++)
         charge_line<T, W>(p, cur, nxt, 0, e1, e2, xh, q, (uint32_t)z, k1, k2, cnt, dSd, dSu);
