@@ -144,8 +144,12 @@ int lgv_measure(lgv_ctx *ctx, int replica, double *wall, double *wallphase,
 
 /* HIP-event timing of the kernel classes of a step on the context's stream
  * (the reference's PetscLogEvent "IdealStep" / "HBStep" / "QHBStep",
- * Stepper.cxx:547-577).  Enabling resets the totals; while enabled, lgv_run
- * issues kernels directly instead of replaying a graph. */
+ * Stepper.cxx:547-577).  enable: 0 off, 1 every step, k > 1 every k-th step
+ * of lgv_step / lgv_run from the next one on (the other steps run as without
+ * timing, the charge sweep beside the phi sweeps where the context does so;
+ * a timed step keeps one stream).  Enabling resets the totals; while
+ * enabled, lgv_run issues kernels directly instead of replaying a graph.
+ * The single sweeps (lgv_ideal_step, ...) are timed whenever enabled. */
 enum { LGV_CLASS_IDEAL = 0, LGV_CLASS_PHI = 1, LGV_CLASS_CHARGE = 2, LGV_CLASS_ENERGY = 3,
        LGV_NCLASS = 4 };
 int lgv_timing_enable(lgv_ctx *ctx, int enable);

@@ -119,7 +119,10 @@ struct lgv_ctx {
   hipGraphExec_t graph_one = nullptr;   // one step (the remainder of a run)
   double graph_dt = NAN;
   // HIP-event timing of kernel classes on the context stream
-  bool timing = false;
+  bool timing = false;      // HIP-event timing configured (lgv_timing_enable)
+  bool timing_now = false;  // the launches being enqueued are timed
+  int timing_every = 1;     // time every k-th step of lgv_step / lgv_run
+  uint64_t timing_ctr = 0;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pairs;
@@ -286,17 +289,17 @@ struct Timed {
   int cls;
   hipEvent_t e1 = nullptr;
   Timed(lgv_ctx *cc, int k) : c(cc), cls(k) {
-    if (!c->timing) return;
+    if (!c->timing_now) return;
     if (c->ev_used + 2 > c->ev_pool.size()) {
       for (int i = 0; i < 64; i++) {
         hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) { c->timing = false; return; }
+        if (hipEventCreate(&e) != hipSuccess) { c->timing = c->timing_now = false; return; }
         c->ev_pool.push_back(e);
       }
     }
     hipEvent_t e0 = c->ev_pool[c->ev_used++];
     if (hipEventRecord(e0, c->stream) != hipSuccess) {  // timing is best effort
-      c->timing = false;
+      c->timing = c->timing_now = false;
       return;
     }
     e1 = c->ev_pool[c->ev_used++];
@@ -305,7 +308,7 @@ struct Timed {
   ~Timed() {
     if (!e1) return;
     if (hipEventRecord(e1, c->stream) != hipSuccess) {
-      c->timing = false;
+      c->timing = c->timing_now = false;
       c->ev_pairs.pop_back();  // the pair this object pushed: never resolved
     }
   }
@@ -327,7 +330,7 @@ int drain_timing(lgv_ctx *c) {
 }
 
 void enqueue_ideal(lgv_ctx *c, int off) {
-  if (c->timing && c->ev_used + 16 > 4096) drain_timing(c);
+  if (c->timing_now && c->ev_used + 16 > 4096) drain_timing(c);
   if (c->track_energy) {
     Timed tm(c, LGV_CLASS_ENERGY);
     launch_energy(c->fp32, c->plan_full, c->dP, c->dS, c->eslots, c->stream);
@@ -390,7 +393,9 @@ void enqueue_charge(lgv_ctx *c, int off, hipStream_t st) {
 int enqueue_step(lgv_ctx *c) {
   int off = 0;
   bool charge_queued = false, join_event = false;
-  const bool fork = c->overlap && !c->timing && !c->onlydiffuse && !c->nodiffuse;
+  c->timing_now = c->timing && c->timing_ctr++ % c->timing_every == 0;
+  // a timed step keeps one stream (events on it bracket every launch)
+  const bool fork = c->overlap && !c->timing_now && !c->onlydiffuse && !c->nodiffuse;
   if (!c->onlydiffuse) {
     for (unsigned i1 = 0; i1 < c->s1; i1++) {
       enqueue_ideal(c, off);
@@ -757,6 +762,7 @@ int lgv_ideal_step(lgv_ctx *c, double dt) {
   HIPCHECK(hipSetDevice(c->device));
   int rc = ensure_coeffs(c, c->coef_hb, c->coef_charge, dt);
   if (rc) return rc;
+  c->timing_now = c->timing;  // single sweeps are always timed
   enqueue_ideal(c, 0);
   launch_advance(c->nrep, c->dS, 0., 0, 1, c->stream);
   return check_launch();
@@ -767,6 +773,7 @@ int lgv_hb_phi_step(lgv_ctx *c, double dt) {
   HIPCHECK(hipSetDevice(c->device));
   int rc = ensure_coeffs(c, dt, c->coef_charge, c->coef_ideal);
   if (rc) return rc;
+  c->timing_now = c->timing;  // single sweeps are always timed
   enqueue_hb(c, 0);
   launch_advance(c->nrep, c->dS, 0., 0, 2, c->stream);
   return check_launch();
@@ -777,6 +784,7 @@ int lgv_charge_step(lgv_ctx *c, double dt) {
   HIPCHECK(hipSetDevice(c->device));
   int rc = ensure_coeffs(c, c->coef_hb, dt, c->coef_ideal);
   if (rc) return rc;
+  c->timing_now = c->timing;  // single sweeps are always timed
   enqueue_charge(c, 0, c->stream);
   launch_advance(c->nrep, c->dS, 0., 0, 7, c->stream);
   return check_launch();
@@ -916,10 +924,14 @@ int lgv_measure(lgv_ctx *c, int r, double *wall, double *wallphase, double *oavg
 
 int lgv_timing_enable(lgv_ctx *c, int enable) {
   if (!c) return fail(LGV_ERR_ARG, "null context");
+  if (enable < 0) return fail(LGV_ERR_ARG, "timing_enable %d < 0", enable);
   HIPCHECK(hipSetDevice(c->device));
   int rc = drain_timing(c);
   if (rc) return rc;
   c->timing = enable != 0;
+  c->timing_now = c->timing;
+  c->timing_every = enable > 1 ? enable : 1;
+  c->timing_ctr = 0;
   for (int k = 0; k < LGV_NCLASS; k++) {
     c->t_ms[k] = 0.;
     c->t_launch[k] = 0;

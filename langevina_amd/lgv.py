@@ -312,6 +312,7 @@ class Context:
         return k.value
 
     def timing_enable(self, enable=True):
+        """False: off; True: time every step; k > 1: every k-th step."""
         _check(lib().lgv_timing_enable(self.h, int(enable)))
 
     def timing(self):

@@ -135,5 +135,6 @@ def test_tile_and_graph_setters_reject_a_null_context(L):
     assert lib.lgv_set_ideal_tile(None, 0, 32, 16) == -1
     assert lib.lgv_use_graph(None, 1) == -1
     assert lib.lgv_set_phi_kernel(None, 0) == -1
+    assert lib.lgv_timing_enable(None, 1) == -1
     k = C.c_int(-5)
     assert lib.lgv_get_phi_kernel(None, C.byref(k)) == -1

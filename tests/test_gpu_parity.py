@@ -506,6 +506,34 @@ def test_phi_kernel_switch_keeps_results(L, prec):
                 assert runs[name][1][kind][k] == runs["site"][1][kind][k], (name, kind, k)
 
 
+def test_sampled_timing_counts_every_kth_step(L):
+    """lgv_timing_enable(ctx, k): the kernels of every k-th step (the first
+    after enabling included) are timed, the others run untimed; results are
+    those of an untimed run."""
+    log("sampled timing")
+    N = 16
+    u = np.random.default_rng(3).normal(size=(N, N, N, 10)) * 0.6
+    out = []
+    for k in (0, 3):
+        ctx = L.Context(L.model_params(N, seed=4, **PAR))
+        ctx.set_field(u)
+        ctx.timing_enable(k)
+        ctx.run(0.04, 7)
+        tm = ctx.timing()
+        if k:
+            # steps 0, 3 and 6: 3 ideal substeps, 12 phi half-sweeps and 3
+            # charge directions each
+            assert (tm["ideal"][1], tm["phi"][1], tm["charge"][1]) == (9, 36, 9)
+            assert all(tm[c][0] > 0 for c in ("ideal", "phi", "charge"))
+        else:
+            assert tm["ideal"][1] == tm["phi"][1] == tm["charge"][1] == 0
+        with pytest.raises(L.LgvError, match="timing_enable"):
+            ctx.timing_enable(-1)
+        out.append(ctx.get_field())
+        ctx.close()
+    np.testing.assert_array_equal(out[0], out[1])
+
+
 def test_sweep_tiles_are_checked(L):
     from langevina_amd.autotune import set_sweep_tiles
     ctx = L.Context(L.model_params(16, seed=1, **PAR))
