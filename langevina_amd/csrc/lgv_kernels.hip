@@ -542,6 +542,17 @@ __device__ __forceinline__ void charge_line(const ReplicaParams &p, int cur, int
   }
 }
 
+// Lane 0's value of v, uniform over the wave (v_readfirstlane per word).
+__device__ __forceinline__ float first_lane(float v) {
+  return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v)));
+}
+__device__ __forceinline__ double first_lane(double v) {
+  const long long bits = __double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)bits);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(bits >> 32));
+  return __longlong_as_double(((long long)hi << 32) | lo);
+}
+
 // The x direction with a whole row in one wavefront or in one aligned
 // segment of seg lanes of it (NH = seg H; seg = NH <= 64 a power of two for
 // H = 1, seg = 64 for H = 2): lane l of the segment owns the second-parity
@@ -564,37 +575,34 @@ __device__ __forceinline__ void charge_row_wave(const ReplicaParams &p, int cur,
   const ChargeK ck = charge_k(p.rms, p.chi);
   const FieldBufs fb = field_bufs<T>(p);
   // a1: updated A'_h; r: updated L_{h+1} from lane + 1; l0: lane 0's updated
-  // L (H = 2 only)
-  double a1[H][6], r[H][6], l0[H][6];
+  // L (H = 2 only); all as stored (T), which is how the second sub-sweep reads
+  // them, so that FP32 fields hold and shuffle half the registers
+  T a1[H][6], r[H][6], l0[H][6];
 #pragma unroll
   for (int j = 0; j < H; j++) {
     const uint32_t h = lane + 64 * j;
     const uint32_t x = site_x(h, y, z, e2);
     const uint32_t xl = x == 0 ? N - 1 : x - 1;
     const uint32_t iA = row * NH + h, iL = row * NH + (xl >> 1);
-    double wl[6], wa[6];
+    T wl[6], wa[6];  // as stored: converted where used
 #pragma unroll
     for (int L = 4; L < 10; L++) {
-      wl[L - 4] = gplane<T, W>(fb, cur, L, e1).ld(iL);
-      wa[L - 4] = gplane<T, W>(fb, cur, L, e2).ld(iA);
+      wl[L - 4] = gplane<T, W>(fb, cur, L, e1).raw(iL);
+      wa[L - 4] = gplane<T, W>(fb, cur, L, e2).raw(iA);
     }
     Xoro gL = xoro_seeded(site_key(k1, row * N + xl));
 #pragma unroll
     for (int c = 0; c < 6; c++) {
       double tl, ta;
-      charge_pair(wl[c], wa[c], ck, gL, tl, ta, cnt, dSd, dSu, true);
+      charge_pair((double)wl[c], (double)wa[c], ck, gL, tl, ta, cnt, dSd, dSu, true);
       // the first sub-sweep's results as stored (rounded in FP32 storage)
-      tl = (double)(T)tl;
-      a1[j][c] = (double)(T)ta;
+      const T tls = (T)tl;
+      a1[j][c] = (T)ta;
       if constexpr (H == 1) {
-        r[j][c] = __shfl(tl, (lane + 1) & (seg - 1), seg);
+        r[j][c] = __shfl(tls, (lane + 1) & (seg - 1), seg);
       } else {
-        r[j][c] = __shfl(tl, (lane + 1) & 63, 64);
-        // lane 0's value, broadcast through scalar registers
-        const long long bits = __double_as_longlong(tl);
-        const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)bits);
-        const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(bits >> 32));
-        l0[j][c] = __longlong_as_double(((long long)hi << 32) | lo);
+        r[j][c] = __shfl(tls, (lane + 1) & 63, 64);
+        l0[j][c] = first_lane(tls);  // lane 0's value, through scalar registers
       }
     }
   }
@@ -607,9 +615,9 @@ __device__ __forceinline__ void charge_row_wave(const ReplicaParams &p, int cur,
 #pragma unroll
     for (int c = 0; c < 6; c++) {
       // H = 2: lane 63's B'_h is lane 0's L of the next j (wrapping to j = 0)
-      const double nb = (H > 1 && lane == 63) ? l0[(j + 1) % H][c] : r[j][c];
+      const double nb = (double)((H > 1 && lane == 63) ? l0[(j + 1) % H][c] : r[j][c]);
       double a2, b2;
-      charge_pair(a1[j][c], nb, ck, gA, a2, b2, cnt, dSd, dSu, true);
+      charge_pair((double)a1[j][c], nb, ck, gA, a2, b2, cnt, dSd, dSu, true);
       gplane<T, W>(fb, nxt, 4 + c, e2).st(row * NH + h, a2);
       gplane<T, W>(fb, nxt, 4 + c, e1).st(row * NH + (xb >> 1), b2);
     }
@@ -750,7 +758,8 @@ __device__ __forceinline__ void charge_column(const ReplicaParams &p, int cur, i
 }
 
 template <typename T, bool W, bool XW>
-__global__ __launch_bounds__(kBlock) void k_charge_march(const ReplicaParams *__restrict__ P,
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && XW ? 4 : 1))) void k_charge_march(const ReplicaParams *__restrict__ P,
                                                          const ReplicaState *__restrict__ S,
                                                          const int *__restrict__ blk_rep,
                                                          const int *__restrict__ blk_first,
