@@ -158,6 +158,12 @@ __device__ void block_monitor_add(MonSlot *slot, uint64_t cnt, double dSd, doubl
 // or above the upper bound does not: the margin, about 1e-3 exp(x), exceeds
 // the 2.4e-10 of the draw's rounding whenever exp(x) > 2.4e-7.  The double
 // draw and the double exp decide the rest (about 2e-3 of the tests).
+// The double test, out of line: it runs for about 2e-3 of the tests, and
+// inlined at every decision of a sweep kernel its exp makes up a sizeable
+// part of the kernel's code.
+__device__ __attribute__((noinline)) bool accept_exact(uint64_t u, double x) {
+  return canonical(u) < exp(x);
+}
 __device__ __forceinline__ bool accept_draw(uint64_t u, double x) {
   if (x > -11.) {
     const float e = __expf((float)x);
@@ -165,7 +171,7 @@ __device__ __forceinline__ bool accept_draw(uint64_t u, double x) {
     if (r < e * (0.999f * 0x1p32f)) return true;
     if (r >= e * (1.001f * 0x1p32f)) return false;
   }
-  return canonical(u) < exp(x);
+  return accept_exact(u, x);
 }
 
 // The Metropolis decision of one phi site (Stepper.cxx:330-370) in the
