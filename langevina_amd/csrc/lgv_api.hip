@@ -329,24 +329,30 @@ int drain_timing(lgv_ctx *c) {
   return LGV_OK;
 }
 
-void enqueue_ideal(lgv_ctx *c, int off) {
+// The launches of a step read the live buffers of the replica state XOR
+// `flip` (bit 0 phi, bit 1 charges), the flips of the step's earlier
+// launches; k_advance applies the step's net flip at its end, so no launch of
+// a step writes the state's buffer bits.  Each enqueue_* returns the flips
+// of its own launches.
+int enqueue_ideal(lgv_ctx *c, int off, int flip) {
   if (c->timing_now && c->ev_used + 16 > 4096) drain_timing(c);
   if (c->track_energy) {
     Timed tm(c, LGV_CLASS_ENERGY);
-    launch_energy(c->fp32, c->plan_full, c->dP, c->dS, c->eslots, c->stream);
+    launch_energy(c->fp32, c->plan_full, c->dP, c->dS, flip, c->eslots, c->stream);
     launch_energy_finish(c->nrep, c->plan_full, c->dP, c->dS, c->eslots, 0, off,
                          c->mon_ideal, c->stream);
   }
   {
     Timed tm(c, LGV_CLASS_IDEAL);
-    launch_ideal(c->fp32, c->plan_ideal, c->dP, c->dS, c->nrep, c->stream);
+    launch_ideal(c->fp32, c->plan_ideal, c->dP, c->dS, flip, c->stream);
   }
   if (c->track_energy) {
     Timed tm(c, LGV_CLASS_ENERGY);
-    launch_energy(c->fp32, c->plan_full, c->dP, c->dS, c->eslots, c->stream);
+    launch_energy(c->fp32, c->plan_full, c->dP, c->dS, flip ^ 3, c->eslots, c->stream);
     launch_energy_finish(c->nrep, c->plan_full, c->dP, c->dS, c->eslots, 1, off,
                          c->mon_ideal, c->stream);
   }
+  return 3;  // out of place: phi and charges
 }
 
 // EulerLangevinHB::step (counters off, off+1 for the two parities), one
@@ -360,25 +366,30 @@ void enqueue_ideal(lgv_ctx *c, int off) {
 // march's lower traffic speeds up the kernels that follow (kMarchBytesFp32).
 bool use_march(const lgv_ctx *c) { return c->phi_march; }
 const Plan &hb_plan(const lgv_ctx *c) { return use_march(c) ? c->plan_hb : c->plan_half; }
-void enqueue_hb(lgv_ctx *c, int off) {
+int enqueue_hb(lgv_ctx *c, int off, int flip) {
   for (int ieo = 0; ieo < 2; ieo++) {
     Timed tm(c, LGV_CLASS_PHI);
     if (use_march(c))
-      launch_hb_march(c->fp32, c->plan_hb, c->dP, c->dS, ieo, off + ieo, c->mon_phi, c->stream);
+      launch_hb_march(c->fp32, c->plan_hb, c->dP, c->dS, ieo, off + ieo, flip, c->mon_phi,
+                      c->stream);
     else
-      launch_hb_phi(c->fp32, c->plan_half, c->dP, c->dS, ieo, off + ieo, c->mon_phi, c->stream);
+      launch_hb_phi(c->fp32, c->plan_half, c->dP, c->dS, ieo, off + ieo, flip, c->mon_phi,
+                    c->stream);
   }
+  return 0;  // in place
 }
 
 // ModelGChargeHB::step: the shuffle, then per direction both parities in
-// one pass (counters off+1+2*ixyz and off+2+2*ixyz, as the unfused order).
-void enqueue_charge(lgv_ctx *c, int off, hipStream_t st) {
+// one pass (counters off+1+2*ixyz and off+2+2*ixyz, as the unfused order),
+// each launch writing the other charge buffer.
+int enqueue_charge(lgv_ctx *c, int off, int flip, hipStream_t st) {
   launch_shuffle(c->nrep, c->dS, c->dP, off, st);
   for (int ixyz = 0; ixyz < 3; ixyz++) {
     Timed tm(c, LGV_CLASS_CHARGE);
-    launch_charge_march(c->fp32, c->plan_charge, c->dP, c->dS, c->nrep, ixyz,
-                        off + 1 + 2 * ixyz, c->mon_charge, st);
+    launch_charge_march(c->fp32, c->plan_charge, c->dP, c->dS, ixyz, off + 1 + 2 * ixyz,
+                        flip ^ (ixyz & 1 ? 2 : 0), c->mon_charge, st);
   }
+  return 2;  // three flips of the charge buffers
 }
 
 // PV2HBSplit::step (Stepper.cxx:542-580); returns the number of stream
@@ -390,31 +401,33 @@ void enqueue_charge(lgv_ctx *c, int off, hipStream_t st) {
 // result is the sequential one bit for bit; the charge sweep is bound by its
 // arithmetic and the phi sweeps by memory, so the two overlap.  Timing runs
 // stay sequential (events on one stream).
-int enqueue_step(lgv_ctx *c) {
+int enqueue_step(lgv_ctx *c, int &flip) {
   int off = 0;
+  flip = 0;
   bool charge_queued = false, join_event = false;
   c->timing_now = c->timing && c->timing_ctr++ % c->timing_every == 0;
   // a timed step keeps one stream (events on it bracket every launch)
   const bool fork = c->overlap && !c->timing_now && !c->onlydiffuse && !c->nodiffuse;
   if (!c->onlydiffuse) {
     for (unsigned i1 = 0; i1 < c->s1; i1++) {
-      enqueue_ideal(c, off);
+      flip ^= enqueue_ideal(c, off, flip);
       off += 1;
       if (fork && i1 + 1 == c->s1 && hipEventRecord(c->ev_fork, c->stream) == hipSuccess &&
           hipStreamWaitEvent(c->stream2, c->ev_fork, 0) == hipSuccess) {
-        enqueue_charge(c, off + 2 * (int)c->s0, c->stream2);
+        // the phi sweeps that follow do not touch the charge bit of flip
+        flip ^= enqueue_charge(c, off + 2 * (int)c->s0, flip, c->stream2);
         charge_queued = true;
         join_event = hipEventRecord(c->ev_join, c->stream2) == hipSuccess;
       }
       for (unsigned i0 = 0; i0 < c->s0; i0++) {
-        enqueue_hb(c, off);
+        flip ^= enqueue_hb(c, off, flip);
         off += 2;
       }
     }
   }
   if (!c->nodiffuse) {
     if (!charge_queued) {
-      enqueue_charge(c, off, c->stream);
+      flip ^= enqueue_charge(c, off, flip, c->stream);
     } else if (!join_event || hipStreamWaitEvent(c->stream, c->ev_join, 0) != hipSuccess) {
       (void)hipStreamSynchronize(c->stream2);  // join on the host instead
     }
@@ -669,8 +682,9 @@ int lgv_step(lgv_ctx *c, double dt) {
   HIPCHECK(hipSetDevice(c->device));
   int rc = ensure_coeffs(c, dt / (c->s0 * c->s1), dt, dt / (c->s1));
   if (rc) return rc;
-  const int n = enqueue_step(c);
-  launch_advance(c->nrep, c->dS, dt, 0, (uint64_t)n, c->stream);
+  int flip;
+  const int n = enqueue_step(c, flip);
+  launch_advance(c->nrep, c->dS, dt, 0, (uint64_t)n, flip, c->stream);
   return check_launch();
 }
 
@@ -727,8 +741,9 @@ int lgv_run(lgv_ctx *c, double dt, int nsteps) {
   if (rc) return rc;
   if (!c->use_graph || c->timing) {
     for (int i = 0; i < nsteps; i++) {
-      const int n = enqueue_step(c);
-      launch_advance(c->nrep, c->dS, dt, 1, (uint64_t)n, c->stream);
+      int flip;
+      const int n = enqueue_step(c, flip);
+      launch_advance(c->nrep, c->dS, dt, 1, (uint64_t)n, flip, c->stream);
     }
     return check_launch();
   }
@@ -741,8 +756,9 @@ int lgv_run(lgv_ctx *c, double dt, int nsteps) {
       hipGraph_t graph;
       HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       for (int i = 0; i < k; i++) {
-        const int n = enqueue_step(c);
-        launch_advance(c->nrep, c->dS, dt, 1, (uint64_t)n, c->stream);
+        int flip;
+        const int n = enqueue_step(c, flip);
+        launch_advance(c->nrep, c->dS, dt, 1, (uint64_t)n, flip, c->stream);
       }
       HIPCHECK(hipStreamEndCapture(c->stream, &graph));
       HIPCHECK(hipGraphInstantiate(g == 0 ? &c->graph_exec : &c->graph_one, graph, nullptr,
@@ -763,8 +779,8 @@ int lgv_ideal_step(lgv_ctx *c, double dt) {
   int rc = ensure_coeffs(c, c->coef_hb, c->coef_charge, dt);
   if (rc) return rc;
   c->timing_now = c->timing;  // single sweeps are always timed
-  enqueue_ideal(c, 0);
-  launch_advance(c->nrep, c->dS, 0., 0, 1, c->stream);
+  const int flip = enqueue_ideal(c, 0, 0);
+  launch_advance(c->nrep, c->dS, 0., 0, 1, flip, c->stream);
   return check_launch();
 }
 
@@ -774,8 +790,8 @@ int lgv_hb_phi_step(lgv_ctx *c, double dt) {
   int rc = ensure_coeffs(c, dt, c->coef_charge, c->coef_ideal);
   if (rc) return rc;
   c->timing_now = c->timing;  // single sweeps are always timed
-  enqueue_hb(c, 0);
-  launch_advance(c->nrep, c->dS, 0., 0, 2, c->stream);
+  const int flip = enqueue_hb(c, 0, 0);
+  launch_advance(c->nrep, c->dS, 0., 0, 2, flip, c->stream);
   return check_launch();
 }
 
@@ -785,8 +801,8 @@ int lgv_charge_step(lgv_ctx *c, double dt) {
   int rc = ensure_coeffs(c, c->coef_hb, dt, c->coef_ideal);
   if (rc) return rc;
   c->timing_now = c->timing;  // single sweeps are always timed
-  enqueue_charge(c, 0, c->stream);
-  launch_advance(c->nrep, c->dS, 0., 0, 7, c->stream);
+  const int flip = enqueue_charge(c, 0, 0, c->stream);
+  launch_advance(c->nrep, c->dS, 0., 0, 7, flip, c->stream);
   return check_launch();
 }
 

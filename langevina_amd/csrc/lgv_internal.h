@@ -157,23 +157,26 @@ __host__ __device__ inline int measure_chunks(int N) {
 }
 
 // Kernel launchers (lgv_kernels.hip).  fp32 selects the storage type.
+// flip: the live-buffer flips of the earlier launches of the step, not yet
+// in the replica state (bit 0 phi, bit 1 charges); launch_advance applies a
+// step's net flip.
 void launch_hb_phi(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
-                   int ieo, int ctr_off, MonSlot *mon, hipStream_t st);
+                   int ieo, int ctr_off, int flip, MonSlot *mon, hipStream_t st);
 void launch_hb_march(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
-                     int ieo, int ctr_off, MonSlot *mon, hipStream_t st);
+                     int ieo, int ctr_off, int flip, MonSlot *mon, hipStream_t st);
 void launch_shuffle(int nrep, ReplicaState *S, const ReplicaParams *P, int ctr_off,
                     hipStream_t st);
 void launch_charge_march(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
-                         int nrep, int ixyz, int ctr_off, MonSlot *mon, hipStream_t st);
+                         int ixyz, int ctr_off, int flip, MonSlot *mon, hipStream_t st);
 void launch_ideal(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
-                  int nrep, hipStream_t st);
+                  int flip, hipStream_t st);
 void launch_energy(bool fp32, const Plan &plan, const ReplicaParams *P,
-                   const ReplicaState *S, EnergySlot *slots, hipStream_t st);
+                   const ReplicaState *S, int flip, EnergySlot *slots, hipStream_t st);
 void launch_energy_finish(int nrep, const Plan &plan, const ReplicaParams *P,
                           ReplicaState *S, const EnergySlot *slots, int which,
                           int ctr_off, MonSlot *mon_ideal, hipStream_t st);
 void launch_advance(int nrep, ReplicaState *S, double dt, int advance_time,
-                    uint64_t nctr, hipStream_t st);
+                    uint64_t nctr, int flip, hipStream_t st);
 void launch_set_time(int nrep, int replica, ReplicaState *S, double t, hipStream_t st);
 // Field transfers go through a staging buffer of at most kStageBytes, in
 // slabs of whole z-planes.

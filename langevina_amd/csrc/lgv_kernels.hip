@@ -217,7 +217,8 @@ __global__ __launch_bounds__(kBlock) void k_hb_phi(const ReplicaParams *__restri
                                                    const ReplicaState *__restrict__ S,
                                                    const int *__restrict__ blk_rep,
                                                    const int *__restrict__ blk_first, int ieo,
-                                                   int ctr_off, MonSlot *__restrict__ mon) {
+                                                   int ctr_off, int flip,
+                                                   MonSlot *__restrict__ mon) {
   const int bid = xcd_block();
   const int r = blk_rep[bid];
   const ReplicaParams &p = P[r];
@@ -245,7 +246,7 @@ __global__ __launch_bounds__(kBlock) void k_hb_phi(const ReplicaParams *__restri
                             (z * N + ym) * NH + xh, (zp * N + y) * NH + xh,
                             (zm * N + y) * NH + xh};
     const ReplicaState &s = S[r];
-    const int cur = s.cur_phi;  // phi planes only (the charge bit may be flipping)
+    const int cur = s.cur_phi ^ (flip & 1);  // phi planes only
     const double mass = p.mass0 + p.dmassdt * s.t;
     const double Lambda2 = 0.5 * (2. * 3. + mass);
     Xoro g = xoro_seeded(site_key(stream_key(p.base_key, s.sweep_ctr + ctr_off),
@@ -280,7 +281,8 @@ __global__ __launch_bounds__(kBlock) void k_hb_march(const ReplicaParams *__rest
                                                      const ReplicaState *__restrict__ S,
                                                      const int *__restrict__ blk_rep,
                                                      const int *__restrict__ blk_first, int ieo,
-                                                     int ctr_off, MonSlot *__restrict__ mon) {
+                                                     int ctr_off, int flip,
+                                                     MonSlot *__restrict__ mon) {
   constexpr int kGrid = kHbMarchGrid;
   // values held widened to FP64 (exact for FP32 fields): each is read by up
   // to four sites, so one conversion on the way in replaces four on the way out
@@ -289,7 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_hb_march(const ReplicaParams *__rest
   const int r = blk_rep[bid];
   const ReplicaParams &p = P[r];
   const ReplicaState &s = S[r];
-  const int cur = s.cur_phi;  // phi planes only (the charge bit may be flipping)
+  const int cur = s.cur_phi ^ (flip & 1);  // phi planes only
   const int q = ieo ^ 1;
   const int N = p.N, NH = p.NH;
   int TXH, TY, TZ;
@@ -769,7 +771,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && XW ? 4 : 1))) void k_charge
                                                          const ReplicaState *__restrict__ S,
                                                          const int *__restrict__ blk_rep,
                                                          const int *__restrict__ blk_first,
-                                                         int ixyz, int ctr_off,
+                                                         int ixyz, int ctr_off, int flip,
                                                          MonSlot *__restrict__ mon) {
   const int bid = xcd_block();
   const int r = blk_rep[bid];
@@ -789,7 +791,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && XW ? 4 : 1))) void k_charge
   uint64_t cnt = 0;
   double dSd = 0., dSu = 0.;
   if (xh < (uint32_t)NH && q < (uint32_t)N && t < TXH * TQ) {
-    const int cur = live(s), nxt = cur ^ 2;
+    const int cur = live(s) ^ flip, nxt = cur ^ 2;
     const int dir = s.order[ixyz];
     const int e1 = s.order[3], e2 = s.order[4];
     const uint64_t k1 = stream_key(p.base_key, s.sweep_ctr + ctr_off);
@@ -828,11 +830,6 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && XW ? 4 : 1))) void k_charge
   block_monitor_add(mon + bid, cnt, dSd, dSu);
 }
 
-__global__ void k_flip_chg(ReplicaState *S, int nrep) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < nrep) S[r].cur_chg ^= 1;
-}
-
 // ------------------------------------------------------------------------
 // IdealPV2::step_no_reject (Stepper.cxx:68-176) fused into one pass:
 // rotate by dt/2, kick the charges with the rotated neighbours, rotate by
@@ -855,7 +852,8 @@ __device__ __forceinline__ void load_site(const ReplicaParams &p, int cur, int x
 template <typename T, bool W, bool NTS>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1)))
 void k_ideal_fused(const ReplicaParams *__restrict__ P, const ReplicaState *__restrict__ S,
-                   const int *__restrict__ blk_rep, const int *__restrict__ blk_first) {
+                   const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
+                   int flip) {
   __shared__ double lds[4][ideal_lds(sizeof(T) == 4)];
   // rotated phi of plane z-1 at the thread's own sites: thread-private slots
   __shared__ double lprv[2][4][kBlock];
@@ -863,7 +861,7 @@ void k_ideal_fused(const ReplicaParams *__restrict__ P, const ReplicaState *__re
   const int r = blk_rep[bid];
   // a reference, not a copy: a private copy would be staged in scratch memory
   const ReplicaParams &p = P[r];
-  const int cur = live(S[r]), nxt = cur ^ 3;
+  const int cur = live(S[r]) ^ flip, nxt = cur ^ 3;
   const int N = p.N, NH = p.NH;
   int TXH, TY, TZ;
   ideal_tile(N, p.tile_cap, p.tile_tz, TXH, TY, TZ);
@@ -1089,14 +1087,6 @@ void k_ideal_fused(const ReplicaParams *__restrict__ P, const ReplicaState *__re
   }
 }
 
-__global__ void k_flip(ReplicaState *S, int nrep) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < nrep) {
-    S[r].cur_phi ^= 1;
-    S[r].cur_chg ^= 1;
-  }
-}
-
 // ------------------------------------------------------------------------
 // IdealPV2::computeEnergy (Stepper.cxx:204-268): per-block compensated
 // partial sums of grad2, hEn and nab2.
@@ -1105,11 +1095,11 @@ __global__ __launch_bounds__(kBlock) void k_energy(const ReplicaParams *__restri
                                                    const ReplicaState *__restrict__ S,
                                                    const int *__restrict__ blk_rep,
                                                    const int *__restrict__ blk_first,
-                                                   EnergySlot *__restrict__ slots) {
+                                                   int flip, EnergySlot *__restrict__ slots) {
   const int bid = xcd_block();
   const int r = blk_rep[bid];
   const ReplicaParams p = P[r];
-  const int cur = live(S[r]);
+  const int cur = live(S[r]) ^ flip;
   int64_t idx = (int64_t)(bid - blk_first[r]) * kBlock + threadIdx.x;
   double g2 = 0., hE = 0., n2 = 0.;
   if (idx < 2 * p.half) {
@@ -1204,12 +1194,18 @@ __global__ void k_energy_finish(const ReplicaParams *P, ReplicaState *S,
 }
 
 // ------------------------------------------------------------------------
+// The end of a step (or of a single sweep): the clock, the stream counter
+// and the live buffers.  The kernels of a step take the buffer flips of the
+// launches before them as an argument (flip: bit 0 phi, bit 1 charges, XORed
+// into the state's live bits) and this kernel applies the step's net flip.
 __global__ void k_advance(ReplicaState *S, int nrep, double dt, int advance_time,
-                          uint64_t nctr) {
+                          uint64_t nctr, int flip) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrep) return;
   if (advance_time) S[r].t += dt;
   S[r].sweep_ctr += nctr;
+  S[r].cur_phi ^= flip & 1;
+  S[r].cur_chg ^= (flip >> 1) & 1;
 }
 
 __global__ void k_set_time(ReplicaState *S, int nrep, int replica, double t) {
@@ -1463,15 +1459,15 @@ static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / 
   } while (0)
 
 void launch_hb_phi(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
-                   int ieo, int ctr_off, MonSlot *mon, hipStream_t st) {
+                   int ieo, int ctr_off, int flip, MonSlot *mon, hipStream_t st) {
   LGV_DISPATCH(k_hb_phi, plan.nblocks, st, P, S, plan.d_blk_rep, plan.d_blk_first, ieo,
-               ctr_off, mon);
+               ctr_off, flip, mon);
 }
 
 void launch_hb_march(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
-                     int ieo, int ctr_off, MonSlot *mon, hipStream_t st) {
+                     int ieo, int ctr_off, int flip, MonSlot *mon, hipStream_t st) {
   LGV_DISPATCH_S(k_hb_march, plan.nblocks, st, P, S, plan.d_blk_rep, plan.d_blk_first, ieo,
-               ctr_off, mon);
+               ctr_off, flip, mon);
 }
 
 void launch_shuffle(int nrep, ReplicaState *S, const ReplicaParams *P, int ctr_off,
@@ -1480,13 +1476,13 @@ void launch_shuffle(int nrep, ReplicaState *S, const ReplicaParams *P, int ctr_o
 }
 
 void launch_charge_march(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
-                         int nrep, int ixyz, int ctr_off, MonSlot *mon, hipStream_t st) {
+                         int ixyz, int ctr_off, int flip, MonSlot *mon, hipStream_t st) {
   // the row-per-wavefront x path costs registers (3 waves per SIMD instead of
   // 4), so it is compiled in only for launches whose replicas all use it
   const int *br = plan.d_blk_rep, *bf = plan.d_blk_first;
   const int g = plan.nblocks;
 #define LGV_CHARGE(TT, WW, XX)                                                         \
-  k_charge_march<TT, WW, XX><<<g, kBlock, 0, st>>>(P, S, br, bf, ixyz, ctr_off, mon)
+  k_charge_march<TT, WW, XX><<<g, kBlock, 0, st>>>(P, S, br, bf, ixyz, ctr_off, flip, mon)
   if (plan.xwave) {
     if (fp32) { if (plan.wide) LGV_CHARGE(float, true, true); else LGV_CHARGE(float, false, true); }
     else { if (plan.wide) LGV_CHARGE(double, true, true); else LGV_CHARGE(double, false, true); }
@@ -1495,23 +1491,21 @@ void launch_charge_march(bool fp32, const Plan &plan, const ReplicaParams *P, Re
     else { if (plan.wide) LGV_CHARGE(double, true, false); else LGV_CHARGE(double, false, false); }
   }
 #undef LGV_CHARGE
-  k_flip_chg<<<nblk(nrep, 64), 64, 0, st>>>(S, nrep);
 }
 
 void launch_ideal(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
-                  int nrep, hipStream_t st) {
-  LGV_DISPATCH_S(k_ideal_fused, plan.nblocks, st, P, S, plan.d_blk_rep, plan.d_blk_first);
-  k_flip<<<nblk(nrep, 64), 64, 0, st>>>(S, nrep);
+                  int flip, hipStream_t st) {
+  LGV_DISPATCH_S(k_ideal_fused, plan.nblocks, st, P, S, plan.d_blk_rep, plan.d_blk_first, flip);
 }
 
 void launch_energy(bool fp32, const Plan &plan, const ReplicaParams *P,
-                   const ReplicaState *S, EnergySlot *slots, hipStream_t st) {
+                   const ReplicaState *S, int flip, EnergySlot *slots, hipStream_t st) {
   if (fp32)
     k_energy<float><<<plan.nblocks, kBlock, 0, st>>>(P, S, plan.d_blk_rep, plan.d_blk_first,
-                                                     slots);
+                                                     flip, slots);
   else
     k_energy<double><<<plan.nblocks, kBlock, 0, st>>>(P, S, plan.d_blk_rep, plan.d_blk_first,
-                                                      slots);
+                                                      flip, slots);
 }
 
 void launch_energy_finish(int nrep, const Plan &plan, const ReplicaParams *P,
@@ -1522,8 +1516,8 @@ void launch_energy_finish(int nrep, const Plan &plan, const ReplicaParams *P,
 }
 
 void launch_advance(int nrep, ReplicaState *S, double dt, int advance_time, uint64_t nctr,
-                    hipStream_t st) {
-  k_advance<<<nblk(nrep, 64), 64, 0, st>>>(S, nrep, dt, advance_time, nctr);
+                    int flip, hipStream_t st) {
+  k_advance<<<nblk(nrep, 64), 64, 0, st>>>(S, nrep, dt, advance_time, nctr, flip);
 }
 
 void launch_set_time(int nrep, int replica, ReplicaState *S, double t, hipStream_t st) {
