@@ -379,11 +379,11 @@ int enqueue_hb(lgv_ctx *c, int off, int flip) {
   return 0;  // in place
 }
 
-// ModelGChargeHB::step: the shuffle, then per direction both parities in
-// one pass (counters off+1+2*ixyz and off+2+2*ixyz, as the unfused order),
-// each launch writing the other charge buffer.
+// ModelGChargeHB::step: the shuffle (counter off, drawn by every charge
+// launch), then per direction both parities in one pass (counters
+// off+1+2*ixyz and off+2+2*ixyz, as the unfused order), each launch writing
+// the other charge buffer.
 int enqueue_charge(lgv_ctx *c, int off, int flip, hipStream_t st) {
-  launch_shuffle(c->nrep, c->dS, c->dP, off, st);
   for (int ixyz = 0; ixyz < 3; ixyz++) {
     Timed tm(c, LGV_CLASS_CHARGE);
     launch_charge_march(c->fp32, c->plan_charge, c->dP, c->dS, ixyz, off + 1 + 2 * ixyz,
@@ -528,8 +528,6 @@ int lgv_create(const lgv_config *cfg, lgv_ctx **out) {
     c->noise[r].g = xoro_seeded((uint64_t)s32);
     hs[r] = ReplicaState{};
     hs[r].t = m.initialtime;
-    hs[r].order[0] = 0; hs[r].order[1] = 1; hs[r].order[2] = 2;
-    hs[r].order[3] = 0; hs[r].order[4] = 1;
     whalf.push_back(half);
     wfull.push_back(2 * half);
     if ((uint64_t)half * esz > 0xFFFFFFFFull) c->wide = true;

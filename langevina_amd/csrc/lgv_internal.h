@@ -37,7 +37,6 @@ struct ReplicaParams {
 struct ReplicaState {
   double t;
   uint64_t sweep_ctr;
-  int order[5];      // charge sub-sweep order: directions[3], parities[2]
   int cur_phi;       // buffer (0/1) holding the live phi planes
   int cur_chg;       // buffer (0/1) holding the live charge planes
   int pad;
@@ -164,8 +163,6 @@ void launch_hb_phi(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaS
                    int ieo, int ctr_off, int flip, MonSlot *mon, hipStream_t st);
 void launch_hb_march(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
                      int ieo, int ctr_off, int flip, MonSlot *mon, hipStream_t st);
-void launch_shuffle(int nrep, ReplicaState *S, const ReplicaParams *P, int ctr_off,
-                    hipStream_t st);
 void launch_charge_march(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
                          int ixyz, int ctr_off, int flip, MonSlot *mon, hipStream_t st);
 void launch_ideal(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,

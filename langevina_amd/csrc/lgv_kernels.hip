@@ -431,18 +431,6 @@ __global__ __launch_bounds__(kBlock) void k_hb_march(const ReplicaParams *__rest
 
 
 // ------------------------------------------------------------------------
-// The shuffled sub-sweep order of ModelGChargeHB::step (Stepper.cxx:461-468)
-// from the replica's per-step shuffle stream.
-__global__ void k_shuffle(ReplicaState *S, const ReplicaParams *P, int nrep, int ctr_off) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrep) return;
-  Xoro g = xoro_seeded(site_key(stream_key(P[r].base_key, S[r].sweep_ctr + ctr_off), 0));
-  int order[5];
-  shuffle_orders(g, order);
-  for (int i = 0; i < 5; i++) S[r].order[i] = order[i];
-}
-
-// ------------------------------------------------------------------------
 // modelg_update_charge_pair (Stepper.cxx:419-441) for one component of one
 // pair, applied as nA + (-q), nB + q; counts the decision when count.
 // The constants of a charge sweep: the proposal width and the divisor 2 chi
@@ -792,8 +780,16 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && XW ? 4 : 1))) void k_charge
   double dSd = 0., dSu = 0.;
   if (xh < (uint32_t)NH && q < (uint32_t)N && t < TXH * TQ) {
     const int cur = live(s) ^ flip, nxt = cur ^ 2;
-    const int dir = s.order[ixyz];
-    const int e1 = s.order[3], e2 = s.order[4];
+    // the shuffled sub-sweep order of ModelGChargeHB::step
+    // (Stepper.cxx:461-468) from the sweep's shuffle stream, the counter
+    // before the first direction's two; every thread draws the same order
+    int order[5];
+    {
+      Xoro g = xoro_seeded(site_key(stream_key(p.base_key, s.sweep_ctr + ctr_off - 1 - 2 * ixyz), 0));
+      shuffle_orders(g, order);
+    }
+    const int dir = order[ixyz];
+    const int e1 = order[3], e2 = order[4];
     const uint64_t k1 = stream_key(p.base_key, s.sweep_ctr + ctr_off);
     const uint64_t k2 = stream_key(p.base_key, s.sweep_ctr + ctr_off + 1);
     if (XW && dir == 0) {  // every replica of the launch has NH = 64 or 128
@@ -1468,11 +1464,6 @@ void launch_hb_march(bool fp32, const Plan &plan, const ReplicaParams *P, Replic
                      int ieo, int ctr_off, int flip, MonSlot *mon, hipStream_t st) {
   LGV_DISPATCH_S(k_hb_march, plan.nblocks, st, P, S, plan.d_blk_rep, plan.d_blk_first, ieo,
                ctr_off, flip, mon);
-}
-
-void launch_shuffle(int nrep, ReplicaState *S, const ReplicaParams *P, int ctr_off,
-                    hipStream_t st) {
-  k_shuffle<<<nblk(nrep, 64), 64, 0, st>>>(S, P, nrep, ctr_off);
 }
 
 void launch_charge_march(bool fp32, const Plan &plan, const ReplicaParams *P, ReplicaState *S,
