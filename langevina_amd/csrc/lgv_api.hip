@@ -357,9 +357,10 @@ int enqueue_ideal(lgv_ctx *c, int off, int flip) {
 
 // EulerLangevinHB::step (counters off, off+1 for the two parities), one
 // in-place launch per parity: the z-march with the other parity's plane in
-// LDS for FP64 fields above 1 GiB (non-temporal mode) and FP32 fields above
-// 600 MiB, the one-thread-per-site kernel otherwise.  Measured per
-// half-sweep, FP64: 256^3 march 157 us against 204 us per site; 192^3 85
+// LDS or the one-thread-per-site kernel, by the size rule (the march for
+// FP64 fields above 1 GiB and FP32 fields above 600 MiB) unless
+// lgv_set_phi_kernel chose.  Measured per half-sweep before the march's
+// prefetch, FP64: 256^3 march 157 us against 204 us per site; 192^3 85
 // against 75 (70 in plane bands), 128^3 35 against 29 (26), 64^3 30 against
 // 9 (below 1 GiB the fields stay largely cache-resident and the per-site
 // kernel's 8 waves per SIMD win); FP32 256^3 110 against 109, but the
@@ -398,9 +399,10 @@ int enqueue_charge(lgv_ctx *c, int off, int flip, hipStream_t st) {
 // sweep runs on the second stream while the phi sweeps run on the first
 // (fork and join by events, which a graph capture records as well).  Every
 // sweep keeps its counter (random streams) and its monitor slots, so the
-// result is the sequential one bit for bit; the charge sweep is bound by its
-// arithmetic and the phi sweeps by memory, so the two overlap.  Timing runs
-// stay sequential (events on one stream).
+// result is the sequential one bit for bit.  Contexts of one lattice whose
+// fields exceed 600 MiB keep the sequential order (their launches fill the
+// GPU on their own), and so do timed steps (events on one stream).  flip
+// returns the step's net live-buffer flip.
 int enqueue_step(lgv_ctx *c, int &flip) {
   int off = 0;
   flip = 0;
