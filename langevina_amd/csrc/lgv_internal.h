@@ -136,9 +136,11 @@ constexpr uint64_t kStreamBytes = 1ull << 30;
 // (256^3: 110 against 109 us), but it moves 1.09x instead of 1.36x the
 // algorithmic bytes and leaves more of the charge and phi planes in the
 // 256 MB last-level cache for the kernels that follow (256^3 FP32 step
-// 3.72 -> 3.61 ms: charge 337 -> 317 us, ideal 429 -> 414 us).  At 192^3
-// FP64 (540 MiB), 128^3 and the 96-replica FP32 batch the per-site kernel
-// gives the faster step.
+// 3.72 -> 3.61 ms: charge 337 -> 317 us, ideal 429 -> 414 us).  Below it
+// the size rule takes the per-site kernel (single 128^3-192^3 lattices run
+// faster with it); the autotuner may choose the march instead (the
+// 96-replica FP32 batch does).  A context of one lattice above this size
+// also keeps the charge sweep after the phi sweeps rather than beside them.
 constexpr uint64_t kMarchBytesFp32 = 600ull << 20;
 
 // Largest side: N^3 must fit 32-bit site indices.
