@@ -72,7 +72,10 @@ def test_hb_phi_sweep_bit_exact(L, orc, N, prec):
     ctx.close()
 
 
-@pytest.mark.parametrize("N,prec", [(8, 0), (16, 0), (16, 1), (32, 0)])
+# N = 4 ... 64: the x sweep in row segments of NH lanes (NH = 2 ... 32); 6
+# and 12 (NH = 3, 6): the x sweep in charge_line
+@pytest.mark.parametrize("N,prec", [(4, 0), (6, 0), (8, 0), (12, 1), (16, 0), (16, 1), (32, 0),
+                                    (64, 1)])
 def test_charge_sweep_bit_exact(L, orc, N, prec):
     log("charge", N, prec)
     ctx, lat = make_pair(L, orc, N, precision=prec, seed=7)
