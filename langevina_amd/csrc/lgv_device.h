@@ -203,11 +203,11 @@ LGV_HD int site_x(int xh, int y, int z, int p) { return 2 * xh + ((p + y + z) & 
 // rest of the library is built with -ffp-contract=off).
 LGV_HD void cos_sinc(double y, double &c, double &s) {
 #pragma clang fp contract(fast)
-  if (y < 0x1p-10) {
+  if (__builtin_expect(y < 0x1p-10, 1)) {
     // the first omitted terms, y^5/10! and y^5/11!, are below 3e-22
     c = 1.0 + y * (-1.0 / 2 + y * (1.0 / 24 + y * (-1.0 / 720 + y * (1.0 / 40320))));
     s = 1.0 + y * (-1.0 / 6 + y * (1.0 / 120 + y * (-1.0 / 5040 + y * (1.0 / 362880))));
-  } else if (y < 0.25) {
+  } else if (__builtin_expect(y < 0.25, 1)) {
     c = 1.0 + y * (-1.0 / 2 + y * (1.0 / 24 + y * (-1.0 / 720 + y * (1.0 / 40320 +
         y * (-1.0 / 3628800 + y * (1.0 / 479001600 + y * (-1.0 / 87178291200.0)))))));
     s = 1.0 + y * (-1.0 / 6 + y * (1.0 / 120 + y * (-1.0 / 5040 + y * (1.0 / 362880 +
