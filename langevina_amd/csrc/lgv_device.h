@@ -108,8 +108,21 @@ LGV_HD double canonical(uint64_t x) {
 // (exact doubling); NoiseGenerator::variance1, NoiseGenerator.h:68-69,105
 constexpr double kSqrt3 = 0x1.bb67ae8584caap+0;  // sqrt(3.) rounded
 LGV_HD double variance1(Xoro &g) {
+#ifdef __HIP_DEVICE_COMPILE__
+  // The same number with the scaling moved into the multiply: d = double(x)
+  // (hi 2^32 + lo, an FMA whose product and addend are exact, one
+  // rounding), clamped at (1 - 2^-53) 2^64, times 2 sqrt(3) 2^-64 (exact):
+  // d 2^-64 is canonical(x) exactly and a product's rounding commutes with
+  // scaling by a power of two (no value here is subnormal).
+  const uint64_t x = xoro_next(g);
+  uint32_t h = (uint32_t)(x >> 32);
+  __asm__("" : "+v"(h));
+  const double d = fmin(fma((double)h, 0x1p32, (double)(uint32_t)x), 0x1.fffffffffffffp63);
+  return d * ((kSqrt3 + kSqrt3) * 0x1p-64) + (-kSqrt3);
+#else
   const double u = canonical(xoro_next(g));
   return u * (kSqrt3 + kSqrt3) + (-kSqrt3);
+#endif
 }
 // uniform_real_distribution(0, 1): u*(1-0)+0 == u; NoiseGenerator::uniform
 LGV_HD double uniform01(Xoro &g) { return canonical(xoro_next(g)); }

@@ -165,12 +165,17 @@ __device__ __attribute__((noinline)) bool accept_exact(uint64_t u, double x) {
   return canonical(u) < exp(x);
 }
 __device__ __forceinline__ bool accept_draw(uint64_t u, double x) {
-  if (x > -11.) {
-    const float e = __expf((float)x);
-    const float r = (float)(uint32_t)(u >> 32);
-    if (r < e * (0.999f * 0x1p32f)) return true;
-    if (r >= e * (1.001f * 0x1p32f)) return false;
-  }
+  // the bounds are formed for every x (a very negative x gives e = 0, no
+  // NaN) and the test falls back to the double path outside x > -11; the
+  // empty asm keeps the draw's top half a 32-bit value (otherwise it is
+  // converted as a 64-bit one, six instructions instead of one)
+  uint32_t h = (uint32_t)(u >> 32);
+  __asm__("" : "+v"(h));
+  const float e = __expf((float)x);
+  const float r = (float)h;
+  const bool below = r < e * (0.999f * 0x1p32f);
+  const bool above = r >= e * (1.001f * 0x1p32f);
+  if (x > -11. && (below || above)) return below;
   return accept_exact(u, x);
 }
 
