@@ -94,6 +94,32 @@ def test_variance1_is_unit_variance(orc):
     assert x.min() >= -3**0.5 and x.max() < 3**0.5
 
 
+def test_variance1_scaling_folded_into_the_multiply(orc):
+    # The device forms variance1 as min(double(x), (1 - 2^-53) 2^64) *
+    # (2 sqrt(3) 2^-64) - sqrt(3) instead of canonical(x) * 2 sqrt(3) -
+    # sqrt(3): the same double for every x, since a product's rounding
+    # commutes with scaling by a power of two.  Draws of every magnitude,
+    # halfway cases of the conversion and the clamp region.
+    import random
+    k = float.fromhex("0x1.bb67ae8584caap+0")
+    kk = (k + k) * 2.0 ** -64
+    cap = float.fromhex("0x1.fffffffffffffp63")
+    rng = random.Random(5)
+    xs = [0, 1, 2, 3, 2**53 + 1, 2**63, 2**64 - 1, 2**64 - 2**10, 2**64 - 2**11,
+          2**64 - 2**11 - 1, 2**64 - 3 * 2**10]
+    for _ in range(20000):
+        b = rng.randrange(1, 65)
+        xs.append(rng.getrandbits(b))
+        # a halfway case of the 53-bit conversion at this magnitude
+        if b > 54:
+            xs.append((rng.getrandbits(b) >> (b - 54) | 1) << (b - 54))
+    for x in xs:
+        x &= 2**64 - 1
+        host = orc.canonical(x) * (k + k) + (-k)
+        dev = min(float(x), cap) * kk + (-k)
+        assert host == dev, hex(x)
+
+
 def test_uniform_int_lemire_rejection(orc):
     # uniform_int{0,b} draws again when the low word falls below
     # (2^64 - range) % range; for b = 5 the threshold is 4.
