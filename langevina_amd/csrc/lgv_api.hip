@@ -761,9 +761,15 @@ int lgv_run(lgv_ctx *c, double dt, int nsteps) {
         launch_advance(c->nrep, c->dS, dt, 1, (uint64_t)n, flip, c->stream);
       }
       HIPCHECK(hipStreamEndCapture(c->stream, &graph));
-      HIPCHECK(hipGraphInstantiate(g == 0 ? &c->graph_exec : &c->graph_one, graph, nullptr,
-                                   nullptr, 0));
+      // the captured graph is released whether or not it instantiates
+      const hipError_t ei = hipGraphInstantiate(g == 0 ? &c->graph_exec : &c->graph_one,
+                                                graph, nullptr, nullptr, 0);
       HIPCHECK(hipGraphDestroy(graph));
+      if (ei != hipSuccess) {
+        (g == 0 ? c->graph_exec : c->graph_one) = nullptr;
+        drop_graphs(c);
+        return fail(LGV_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+      }
     }
     c->graph_dt = dt;
   }
