@@ -373,41 +373,58 @@ __global__ __launch_bounds__(kBlock) void k_hb_march(const ReplicaParams *__rest
 #pragma unroll
     for (int l = 0; l < 4; l++) go[l].template put_raw<NTS>(i, v[l]);
   };
+  // FP32 fields: the loads run two planes ahead in two register sets (a
+  // set holds half the bytes of an FP64 one, so one set in flight per wave
+  // leaves the memory system short of requests; 256^3 half-sweep 94 -> 92
+  // us, 119 registers, still four waves per SIMD); FP64: one plane ahead.
+  constexpr int kDepth = sizeof(T) == 4 ? 2 : 1;
   double qm[4], qc[4], qp[4];  // q values of the own half-column at z-1, z, z+1
-  T nq[4], nphi[4], nh[4];
+  // set s: the own q values at z+1, the own update-parity value at z and the
+  // ring entry at z, for the planes z = z0 + s, z0 + s + kDepth, ...
+  T nq[kDepth][4], nphi[kDepth][4], nh[kDepth][4];
   T pend[4];         // plane z-1's new values, stored at the top of iteration z
   uint32_t pend_i = 0;
   if (own) {
     ld4(idx(xh, y, wrapz(z0 - 1, N)), qm);
     ld4(idx(xh, y, z0), qc);
-    raw4(idx(xh, y, z0 + 1 == N ? 0 : z0 + 1), nq);
-    own4(idx(xh, y, z0), nphi);
   }
-  if (halo) raw4(idx(hgxh, hgy, z0), nh);
-  for (int j = 0; j < tz; j++) {
+#pragma unroll
+  for (int sidx = 0; sidx < kDepth; sidx++) {
+    if (sidx < tz) {  // plane z0 + sidx < z0 + tz <= N
+      const int zs = z0 + sidx;
+      if (own) {
+        raw4(idx(xh, y, zs + 1 == N ? 0 : zs + 1), nq[sidx]);
+        own4(idx(xh, y, zs), nphi[sidx]);
+      }
+      if (halo) raw4(idx(hgxh, hgy, zs), nh[sidx]);
+    }
+  }
+  // plane z0 + j from register set sq / sphi / sh, which then takes the
+  // loads of plane z0 + j + kDepth
+  auto plane = [&](int j, T (&sq)[4], T (&sphi)[4], T (&sh)[4]) __attribute__((always_inline)) {
     const int z = z0 + j;
-    const int zp = z + 1 == N ? 0 : z + 1;
     double(*g)[kGrid] = Q[j & 1];
     double phi_o[4];
     if (own) {
       const int pos = (ty + 1) * GW + (tx + 1);
 #pragma unroll
       for (int l = 0; l < 4; l++) {
-        qp[l] = nq[l];
-        phi_o[l] = nphi[l];
+        qp[l] = sq[l];
+        phi_o[l] = sphi[l];
         g[l][pos] = qc[l];
       }
     }
     if (halo) {
 #pragma unroll
-      for (int l = 0; l < 4; l++) g[l][hpos] = nh[l];
+      for (int l = 0; l < 4; l++) g[l][hpos] = sh[l];
     }
-    if (j + 1 < tz) {  // plane z+1 (z+1 < z0 + tz <= N)
+    if (j + kDepth < tz) {  // plane zn = z + kDepth < z0 + tz <= N
+      const int zn = z + kDepth;
       if (own) {
-        raw4(idx(xh, y, zp + 1 == N ? 0 : zp + 1), nq);
-        own4(idx(xh, y, zp), nphi);
+        raw4(idx(xh, y, zn + 1 == N ? 0 : zn + 1), sq);
+        own4(idx(xh, y, zn), sphi);
       }
-      if (halo) raw4(idx(hgxh, hgy, zp), nh);
+      if (halo) raw4(idx(hgxh, hgy, zn), sh);
     }
     if (own && j > 0) store4(pend_i, pend);
     __syncthreads();
@@ -428,6 +445,14 @@ __global__ __launch_bounds__(kBlock) void k_hb_march(const ReplicaParams *__rest
         qm[l] = qc[l];
         qc[l] = qp[l];
       }
+    }
+  };
+  // tz is the same for the whole block, so every thread makes the same calls
+  // (and reaches the same barriers)
+  for (int j = 0; j < tz; j += kDepth) {
+    plane(j, nq[0], nphi[0], nh[0]);
+    if constexpr (kDepth == 2) {
+      if (j + 1 < tz) plane(j + 1, nq[kDepth - 1], nphi[kDepth - 1], nh[kDepth - 1]);
     }
   }
   if (own && tz > 0) store4(pend_i, pend);

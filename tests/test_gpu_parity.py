@@ -447,20 +447,23 @@ def test_64bit_offset_kernels_match(L, orc, monkeypatch):
         np.testing.assert_array_equal(val, out["0", "0", key[2]], err_msg=str(key))
 
 
-@pytest.mark.parametrize("N", [48, 128])
-def test_sweep_tiles_do_not_change_results(L, monkeypatch, N):
+@pytest.mark.parametrize("N,prec", [(48, 0), (128, 0), (48, 1), (128, 1)])
+def test_sweep_tiles_do_not_change_results(L, monkeypatch, N, prec):
     """The tiles of the phi z-march and of the charge march (including tiles
     that turn the whole-row x sweep of N = 128 off, ragged tiles at N = 48,
-    and short chunks) give the bits and acceptance counts of the defaults;
-    a retile in the middle of a run keeps the monitor counts."""
-    log("sweep tiles", N)
+    short chunks, and odd z-chunks, which end the FP32 march's two-plane
+    prefetch on one plane) give the bits and acceptance counts of the
+    defaults; a retile in the middle of a run keeps the monitor counts."""
+    log("sweep tiles", N, prec)
     from langevina_amd.autotune import set_sweep_tiles
     monkeypatch.setenv("LGV_PHI_KERNEL", "march")
     u = np.random.default_rng(N + 1).normal(size=(N, N, N, 10)) * 0.6
+    if prec:
+        u = u.astype(np.float32).astype(np.float64)
     ref = None
     for tiles in ((0, 0, 0, 0), (16, 8, 32, 4), (64, 32, 16, 32), (32, 16, 64, 8),
-                  (16, 4, 16, 2)):
-        ctx = L.Context(L.model_params(N, seed=9, **PAR))
+                  (16, 4, 16, 2), (32, 5, 0, 0), (64, 1, 0, 0)):
+        ctx = L.Context(L.model_params(N, seed=9, **PAR), precision=prec)
         ctx.set_field(u)
         ctx.run(0.04, 1)
         set_sweep_tiles(ctx, 0, *tiles)
