@@ -553,8 +553,9 @@ def main():
         "roofline": h["roofline"], "kernels": h["kernels"], "measurement": h["measurement"],
     }
     if not fp32 and not args.headline_only:
-        # the north star's comparison: FP32 fields and the batched ensemble
-        # against this FP64 form, and what FP32 storage does to the observables
+        # the other two forms of the throughput comparison, FP32 fields and
+        # the batched ensemble, against this FP64 form, and what FP32 storage
+        # does to the observables
         f = run_single(L, N, True, args.steps, args.warmup, graph=args.graph,
                        tune=not args.no_autotune)
         progress(f"{f['workload']}: {f['value']:.3e} site-updates/s")
