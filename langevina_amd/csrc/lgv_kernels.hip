@@ -176,7 +176,9 @@ __device__ __forceinline__ bool accept_draw(uint64_t u, double x) {
   const bool below = r < e * (0.999f * 0x1p32f);
   const bool above = r >= e * (1.001f * 0x1p32f);
   if (x > -11. && (below || above)) return below;
-  return accept_exact(u, x);
+  // the draw rebuilt from h (the same bits), so that u's top half need not
+  // be kept beside h
+  return accept_exact(((uint64_t)h << 32) | (uint32_t)u, x);
 }
 
 // The Metropolis decision of one phi site (Stepper.cxx:330-370) in the
