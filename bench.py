@@ -43,12 +43,12 @@ SCHEDULE_BYTES_PER_SITE_UPDATE_FP32 = SCHEDULE_BYTES_PER_SITE_UPDATE
 # copy-kernel calibration (bench.py --calibrate-copy); the summary of those
 # runs is profiles/modelg_256_hbm_traffic.md.
 MEASURED_TRAFFIC = {
-    ("k_ideal_fused", "fp64", 256): (99.48, 80.00),
-    ("k_hb_march", "fp64", 256): (37.94, 16.01),
+    ("k_ideal_fused", "fp64", 256): (99.71, 80.00),
+    ("k_hb_march", "fp64", 256): (37.98, 16.01),
     ("k_charge_march", "fp64", 256): (51.67, 48.25),
-    ("k_ideal_fused", "fp32", 256): (53.30, 40.00),
-    ("k_hb_march", "fp32", 256): (17.72, 8.01),
-    ("k_charge_march", "fp32", 256): (25.76, 24.15),
+    ("k_ideal_fused", "fp32", 256): (53.40, 40.00),
+    ("k_hb_march", "fp32", 256): (17.76, 8.01),
+    ("k_charge_march", "fp32", 256): (25.75, 24.15),
 }
 
 
